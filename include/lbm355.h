@@ -1,0 +1,247 @@
+/*
+ * lbm355.h - C ABI of the MI355X (gfx950) D3Q27 lattice-Boltzmann step that
+ * replaces the lbm3d time step behind solver/sim_tcpc of buresjan/nonnewlbm.
+ *
+ * The reference has no C ABI: its hot path is reached through the sim_tcpc
+ * process CLI (solver/sim_tcpc/sim_tcpc.cu:70-125,846-857) and, inside the
+ * process, through C++ template policies and the State virtual hooks
+ * (solver/lbm3d/state.h:125-211).  Each entry point below names the reference
+ * interface it stands in for.  Plain C types only: pointers, sizes, scalars.
+ *
+ * Conventions
+ *  - Grids are X*Y*Z cells, dense host arrays are x-fastest: i = x + X*(y + Y*z),
+ *    the same order as the legacy-VTK CELL/POINT data the reference reads and
+ *    writes (solver/sim_tcpc/sim_tcpc.cu:176-183, solver/lbm3d/state.hpp:815-818).
+ *  - Direction index q = 9*(cx+1) + 3*(cy+1) + (cz+1), opposite(q) = 26-q
+ *    (solver/lbm3d/defs.h:215-247).
+ *  - Every function returns LBM355_OK (0) or a negative LBM355_E* code and never
+ *    throws; lbm355_last_error() gives a thread-local message for the last failure.
+ *  - A case handle is not thread-safe: drive it from one host thread.
+ */
+#ifndef LBM355_H
+#define LBM355_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM355_ABI_VERSION 1
+
+/* status codes */
+enum {
+    LBM355_OK = 0,
+    LBM355_EINVAL = -1,     /* bad argument or descriptor                       */
+    LBM355_ENOMEM = -2,     /* host or device allocation failed / over budget    */
+    LBM355_EDEVICE = -3,    /* HIP runtime error                                 */
+    LBM355_ETERMINATE = -4, /* case setup found zero inlet/outlet area; the
+                               reference sets lbm.terminate and exits with 2
+                               (solver/sim_tcpc/sim_tcpc.cu:633-640,837)         */
+    LBM355_ENOTFOUND = -5,  /* unknown id (pool)                                 */
+    LBM355_EBUSY = -6       /* request still running (pool poll)                 */
+};
+
+/* collision operators: LBM_TYPE policies of the reference */
+enum {
+    LBM355_CUM = 0, /* LBM_CUM, solver/lbm3d/lbm_col_cum.h:7-455 (sim_tcpc default)  */
+    LBM355_BGK = 1, /* LBM_BGK, solver/lbm3d/lbm_col_bgk.h:9-170                     */
+    LBM355_MRT = 2, /* LBM_MRT ("MRT_LES"), solver/lbm3d/lbm_col_mrt.h:7-78          */
+    LBM355_KBC = 3  /* LBM_KBC_N4, solver/lbm3d/lbm_col_kbc_n.h:1024-1259            */
+};
+
+/* population storage; arithmetic is FP64 for F64 and FP32 for the others */
+enum {
+    LBM355_F64 = 0,         /* TraitsDP (solver/lbm3d/defs.h:141)                */
+    LBM355_F32 = 1,         /* TraitsSP, the sim_tcpc default (defs.h:140)       */
+    LBM355_F16_OFFSET = 2,  /* f_q - w_q stored as IEEE half, FP32 arithmetic    */
+    LBM355_BF16_OFFSET = 3  /* f_q - w_q stored as bfloat16, FP32 arithmetic     */
+};
+
+/* Carreau-Yasuda coupling */
+enum {
+    LBM355_CY_OFF = 0,      /* what a single-rank sim_tcpc run computes: the strain
+                               pre-kernels never run, so S=0, the extra force is 0
+                               and the flow is Newtonian at nu_ref
+                               (solver/lbm3d/core.h:1159-1166,663-781)           */
+    LBM355_CY_NEQ = 1,      /* local nu(gamma) with gamma from the non-equilibrium
+                               second moment of each cell, folded into omega     */
+    LBM355_CY_FD_FORCE = 2  /* the reference multi-rank algorithm: velocity
+                               pre-pass, finite-difference strain, body force
+                               2(nu-nu_ref) div S (core.h:293-384,407-576,663-781) */
+};
+
+/* inflow profile on the IVC/SVC planes */
+enum {
+    LBM355_PROFILE_PLUG = 0,     /* u = flux/area on every open inlet cell
+                                    (solver/sim_tcpc/sim_tcpc.cu:295-309)        */
+    LBM355_PROFILE_PARABOLIC = 1 /* u = u_max (1 - r^2/R^2), r from the inlet axis */
+};
+
+/* Case descriptor.  Everything the reference reads from argv, the material
+ * table and the geometry VTK for one run (sim_tcpc.cu:788-836), in lattice units
+ * except dl.  Zero-initialise and fill; unknown future fields stay zero. */
+typedef struct lbm355_case_desc {
+    int32_t nx, ny, nz;          /* cell counts (VTK dims - 1)                     */
+    const int32_t* geometry;     /* nx*ny*nz input labels, value 1 = wall
+                                    (readGeometry, sim_tcpc.cu:134-185)            */
+    double dl;                   /* physical spacing [m]                           */
+    double dt;                   /* physical time step [s] (sim_tcpc.cu:800)       */
+    int32_t collision;           /* LBM355_CUM ...                                 */
+    int32_t storage;             /* LBM355_F64 ...                                 */
+    double nu_ref;               /* lattice viscosity (--lbm-viscosity)            */
+    /* Carreau-Yasuda in lattice units (sim_tcpc.cu:801-806); cy_enabled=0 for a
+       Newtonian material */
+    int32_t cy_enabled;
+    int32_t cy_model;            /* LBM355_CY_*                                    */
+    double cy_nu0, cy_lambda, cy_a, cy_n;
+    /* inflow: lattice volume fluxes (sim_tcpc.cu:813-814); the IVC (label 3, x=0)
+       moves +x, the SVC (label 4, x=X-1) moves -x (sim_tcpc.cu:657-663) */
+    double ivc_flux, svc_flux;
+    int32_t profile;             /* LBM355_PROFILE_*                               */
+    double ivc_umax, svc_umax;   /* parabolic peak velocities (lattice units)      */
+    double ivc_center[2], svc_center[2]; /* (y,z) axis of each inlet; <0 = centroid */
+    double ivc_radius, svc_radius;       /* parabolic radius; <=0 = from the cells  */
+    /* pulsatile factor u(t) = u (1 + amp sin(2 pi t / period)), t in steps */
+    double pulse_amp, pulse_period;
+    int32_t stats_every;         /* accumulate mean/rms every k steps (1 = every
+                                    step, as sim_tcpc.cu:347-353); 0 = off        */
+    /* boundary kind of the faces x-, x+, y-, y+, z-, z+.  All zero = the sim_tcpc
+       layout (x-: IVC inflow label 3, x+: SVC inflow label 4, y-: outlet label 5,
+       y+: outlet label 6, z: wall; sim_tcpc.cu:524-529).  Otherwise every face is
+       an LBM355_FACE_* value; an outflow on x+ gets label 7, on x- label 8.
+       Faces are labelled x, then y, then z (later faces win on edges), and every
+       non-wall face gets the open-boundary extrusion of sim_tcpc.cu:549-586. */
+    int32_t face_bc[6];
+    int32_t reserved[1];
+} lbm355_case_desc;
+
+/* face boundary kinds for lbm355_case_desc.face_bc */
+enum {
+    LBM355_FACE_TCPC = 0,    /* use the sim_tcpc layout (all six must be 0)        */
+    LBM355_FACE_WALL = 1,
+    LBM355_FACE_INFLOW = 2,  /* x faces only: f_eq(rho from the next plane, u_in)  */
+    LBM355_FACE_OUTFLOW = 3  /* x or y faces: f_eq(1, u from the next plane)        */
+};
+
+/* geometry / footprint report of a created case */
+typedef struct lbm355_case_info {
+    int64_t fluid;       /* label 0 cells                                          */
+    int64_t wall;        /* wall cells (mask or z-plane)                           */
+    int64_t area_ivc, area_svc, area_out5, area_out6;  /* open BC-plane cells       */
+    int64_t input_label3, input_label4, input_label5, input_label6; /* VTK counts   */
+    int64_t active_cells;   /* non-wall cells updated per step                     */
+    int64_t bricks_active;  /* 4x4x4 bricks holding an active cell                 */
+    int64_t bricks_stored;  /* bricks with population storage                      */
+    int64_t device_bytes;   /* device memory held by the case                      */
+    int64_t steps;          /* time steps done                                     */
+    int32_t terminate;      /* 1 if a zero-area inlet/outlet was found             */
+    int32_t reserved[5];
+} lbm355_case_info;
+
+/* dense field ids for lbm355_case_get_field (double, nx*ny*nz, x-fastest) */
+enum {
+    LBM355_FIELD_RHO = 0, LBM355_FIELD_VX = 1, LBM355_FIELD_VY = 2, LBM355_FIELD_VZ = 3,
+    /* statistics sums (MacroTcpc e_vm_*, sim_tcpc.cu:347-353): sum rho*u,
+       sum (rho*u)^2 per component */
+    LBM355_FIELD_SUM_RUX = 4, LBM355_FIELD_SUM_RUY = 5, LBM355_FIELD_SUM_RUZ = 6,
+    LBM355_FIELD_SUM_RRUXX = 7, LBM355_FIELD_SUM_RRUYY = 8, LBM355_FIELD_SUM_RRUZZ = 9,
+    /* strain tensor S11,S12,S13,S22,S32,S33 (lattice units) from the last
+       lbm355_case_compute_strain */
+    LBM355_FIELD_S11 = 10, LBM355_FIELD_S12 = 11, LBM355_FIELD_S13 = 12,
+    LBM355_FIELD_S22 = 13, LBM355_FIELD_S32 = 14, LBM355_FIELD_S33 = 15,
+    LBM355_FIELD_LABEL = 16,     /* map label after setup (VTK "wall")            */
+    LBM355_FIELD_WALLMASK = 17,  /* wall mask after the open-boundary extrusion   */
+    LBM355_FIELD_FX = 18, LBM355_FIELD_FY = 19, LBM355_FIELD_FZ = 20, /* body force */
+    LBM355_FIELD_COUNT = 21
+};
+
+/* --- library --------------------------------------------------------------- */
+const char* lbm355_version(void);
+const char* lbm355_last_error(void);
+/* number of visible HIP devices (0 on a host without a GPU) */
+int lbm355_device_count(int* count);
+
+/* --- one case: replaces StateTcpc + LBM + execute (sim_tcpc.cu:438-838,
+       solver/lbm3d/core.h:1476-1628) ----------------------------------------- */
+typedef struct lbm355_case lbm355_case;
+
+/* Reads the descriptor, applies the sim_tcpc label setup (setupBoundaries +
+ * projectWall, sim_tcpc.cu:494-645, lbm.hpp:145-155), builds the sparse brick
+ * index, uploads f = f_eq(1,0) (resetLattice, sim_tcpc.cu:647-655) and the
+ * initial statistics term (LBMKernelInit, core.h:839-863).  On a zero-area
+ * inlet/outlet the case is still created, info.terminate = 1 and the call
+ * returns LBM355_ETERMINATE (the reference runs and exits with code 2). */
+int lbm355_case_create(const lbm355_case_desc* desc, lbm355_case** out);
+int lbm355_case_destroy(lbm355_case* c);
+int lbm355_case_info_get(const lbm355_case* c, lbm355_case_info* info);
+
+/* per-step inflow hook (updateKernelVelocities, sim_tcpc.cu:657-663): lattice
+   volume fluxes of the IVC and SVC; parabolic profiles scale u_max by the
+   ratio to the fluxes given at create */
+int lbm355_case_set_inflow(lbm355_case* c, double ivc_flux, double svc_flux);
+
+/* n time steps (SimUpdate, core.h:1029-1340, n times); asynchronous on the
+   case's stream; ends with the statistics and macro fields of the last step */
+int lbm355_case_step(lbm355_case* c, int64_t n);
+int lbm355_case_sync(lbm355_case* c);
+
+/* dense double copy of one field; bytes must be >= nx*ny*nz*8 */
+int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes);
+/* populations after the last step in the reference's A/B sense: out[q][cell] =
+   post-collision f_q of each active cell (df_out after SimUpdate); wall cells
+   are 0; bytes >= 27*nx*ny*nz*8 */
+int lbm355_case_get_populations(lbm355_case* c, double* host, size_t bytes);
+
+/* finite-difference strain S of the current velocity field with the reference
+   stencil (cudaLBMKernelStress, core.h:407-576) into the S fields */
+int lbm355_case_compute_strain(lbm355_case* c);
+
+/* signed lattice flux through the four BC planes {IVC, SVC, label5, label6}:
+   sum of u_axis * sign over the plane cells, sign as compute_flows.py:96-104
+   (inlets reported positive inflow, compute_flows.py:133-137); mean=1 uses
+   the accumulated mean momentum instead of the last step */
+int lbm355_case_flux(lbm355_case* c, int mean, double out[4]);
+
+/* zero the statistics sums (statReset, sim_tcpc.cu:665-682) */
+int lbm355_case_reset_stats(lbm355_case* c);
+
+/* reference-layout 3D output: results_<id>/vtk3D/rank000_data_<n>.vtk with the
+   13 arrays of StateTcpc::outputData (sim_tcpc.cu:716-784) through the legacy
+   binary writer (state.hpp:782-851); phys_* describe the material */
+typedef struct lbm355_output_desc {
+    double phys_density;   /* kg/m^3                                              */
+    double phys_nu_inf;    /* m^2/s (physViscosity)                               */
+    double time;           /* physical time written to FIELD TIME                 */
+    int32_t cycle;         /* VTK counter                                         */
+    int32_t stat_steps;    /* iterations - prev_iter (sim_tcpc.cu:720)            */
+    int32_t reserved[6];
+} lbm355_output_desc;
+int lbm355_case_write_vtk(lbm355_case* c, const char* path, const lbm355_output_desc* o);
+
+/* --- batches: many cases stepped by one launch per step ------------------ */
+/* No counterpart in the reference (it runs one case per process,
+   scripts/slurm_run_cases.py:104-198).  A batch steps cases that share
+   collision and storage in one fused launch over the concatenated brick
+   index; graph=1 replays a captured hipGraph of the step. */
+typedef struct lbm355_batch lbm355_batch;
+int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batch** out);
+int lbm355_batch_step(lbm355_batch* b, int64_t n);
+int lbm355_batch_destroy(lbm355_batch* b);
+/* launch shape: bricks per workgroup (1,2,4,8); 0 = autotuned */
+int lbm355_batch_set_shape(lbm355_batch* b, int bricks_per_group);
+int lbm355_batch_autotune(lbm355_batch* b, int* best_bricks_per_group);
+
+/* bytes of device memory a descriptor would need (admission control) */
+int lbm355_case_footprint(const lbm355_case_desc* desc, int64_t* bytes);
+
+/* --- raw kernel timing (measurement only) ---------------------------------- */
+/* average duration in ms of the fused stream-collide launch over n steps,
+   timed with HIP events on the case's stream */
+int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM355_H */

@@ -1,0 +1,1183 @@
+// lbm355_host.cpp - host side of liblbm355: case setup, brick index, C ABI.
+//
+// Replaces the host half of sim_tcpc (StateTcpc::setupBoundaries / resetLattice /
+// updateKernelVelocities / statReset / outputData, solver/sim_tcpc/sim_tcpc.cu:
+// 438-785) and of the lbm3d runtime (LBM container, execute, SimUpdate,
+// solver/lbm3d/lbm.hpp, core.h:1029-1628).  The device work is in
+// lbm355_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/lbm355.h"
+#include "lbm355_internal.h"
+
+using namespace lbm355;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) return fail(LBM355_EDEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_TRY(expr)                                                                \
+    do {                                                                                \
+        int r_ = (expr);                                                                \
+        if (r_ != 0) return fail(LBM355_EDEVICE, std::string(#expr ": ") + kernel_error()); \
+    } while (0)
+
+inline int qx(int q) { return q / 9 - 1; }
+inline int qy(int q) { return (q / 3) % 3 - 1; }
+inline int qz(int q) { return q % 3 - 1; }
+
+size_t storage_bytes(int s) { return s == LBM355_F64 ? 8 : s == LBM355_F32 ? 4 : 2; }
+size_t compute_bytes(int s) { return s == LBM355_F64 ? 8 : 4; }
+
+uint64_t spread3(uint32_t v)
+{
+    uint64_t x = v & 0x1FFFFF;
+    x = (x | x << 32) & 0x1F00000000FFFFull;
+    x = (x | x << 16) & 0x1F0000FF0000FFull;
+    x = (x | x << 8) & 0x100F00F00F00F00Full;
+    x = (x | x << 4) & 0x10C30C30C30C30C3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t n)
+    {
+        bytes = n;
+        if (n == 0) return 0;
+        if (hipMalloc(&p, n) != hipSuccess) { p = nullptr; return -1; }
+        return 0;
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// the label setup of sim_tcpc: every cell fluid, x planes 3/4, y planes 5/6,
+// z planes wall, wall mask from the geometry (value 1), open-boundary
+// extrusion of the wall mask, areas, projectWall.  Generalised to a per-face
+// table; all-zero faces reproduce sim_tcpc.cu:494-645 and lbm.hpp:145-155.
+struct Labels {
+    int X = 0, Y = 0, Z = 0;
+    std::vector<uint8_t> map, wall;
+    int64_t area[4] = {0, 0, 0, 0};  // x-, x+, y-, y+ open BC cells
+    int64_t in_lab[4] = {0, 0, 0, 0};
+    int64_t n_wall = 0, n_fluid = 0;
+    int face[6];
+    bool terminate = false;
+
+    int64_t at(int x, int y, int z) const { return x + int64_t(X) * (y + int64_t(Y) * z); }
+
+    int build(const lbm355_case_desc& d)
+    {
+        X = d.nx; Y = d.ny; Z = d.nz;
+        const int64_t N = int64_t(X) * Y * Z;
+        bool tcpc = true;
+        for (int i = 0; i < 6; ++i) tcpc = tcpc && d.face_bc[i] == LBM355_FACE_TCPC;
+        static const int layout[6] = {LBM355_FACE_INFLOW, LBM355_FACE_INFLOW, LBM355_FACE_OUTFLOW,
+                                      LBM355_FACE_OUTFLOW, LBM355_FACE_WALL, LBM355_FACE_WALL};
+        for (int i = 0; i < 6; ++i) {
+            face[i] = tcpc ? layout[i] : d.face_bc[i];
+            if (face[i] < LBM355_FACE_WALL || face[i] > LBM355_FACE_OUTFLOW)
+                return fail(LBM355_EINVAL, "face_bc: mix of 0 and explicit kinds");
+            if (face[i] == LBM355_FACE_INFLOW && i >= 2) return fail(LBM355_EINVAL, "inflow only on x faces");
+            if (face[i] == LBM355_FACE_OUTFLOW && i >= 4) return fail(LBM355_EINVAL, "outflow only on x/y faces");
+        }
+        map.assign(N, L_FLUID);
+        wall.resize(N);
+        for (int64_t i = 0; i < N; ++i) {
+            const int v = d.geometry[i];
+            wall[i] = v == 1;
+            if (v >= 3 && v <= 6) in_lab[v - 3]++;
+        }
+        const uint8_t xm = face[0] == LBM355_FACE_WALL ? L_WALL : face[0] == LBM355_FACE_INFLOW ? L_IN_XM : L_OUT_XM;
+        const uint8_t xp = face[1] == LBM355_FACE_WALL ? L_WALL : face[1] == LBM355_FACE_INFLOW ? L_IN_XP : L_OUT_XP;
+        const uint8_t ym = face[2] == LBM355_FACE_WALL ? L_WALL : L_OUT_YM;
+        const uint8_t yp = face[3] == LBM355_FACE_WALL ? L_WALL : L_OUT_YP;
+        for (int z = 0; z < Z; ++z)
+            for (int y = 0; y < Y; ++y) { map[at(0, y, z)] = xm; map[at(X - 1, y, z)] = xp; }
+        for (int z = 0; z < Z; ++z)
+            for (int x = 0; x < X; ++x) { map[at(x, 0, z)] = ym; map[at(x, Y - 1, z)] = yp; }
+        for (int y = 0; y < Y; ++y)
+            for (int x = 0; x < X; ++x) { map[at(x, y, 0)] = L_WALL; map[at(x, y, Z - 1)] = L_WALL; }
+        // open-boundary extrusion: copy the open pattern of the first non-empty
+        // interior plane outwards to the boundary plane
+        auto open_x = [&](int x) {
+            for (int z = 0; z < Z; ++z)
+                for (int y = 0; y < Y; ++y)
+                    if (!wall[at(x, y, z)]) return true;
+            return false;
+        };
+        auto open_y = [&](int y) {
+            for (int z = 0; z < Z; ++z)
+                for (int x = 0; x < X; ++x)
+                    if (!wall[at(x, y, z)]) return true;
+            return false;
+        };
+        auto extrude_x = [&](int bx, int dir) {
+            int s = bx + dir;
+            while (s > 0 && s < X - 1 && !open_x(s)) s += dir;
+            if (s <= 0 || s >= X - 1) return;
+            for (int z = 0; z < Z; ++z)
+                for (int y = 0; y < Y; ++y)
+                    if (!wall[at(s, y, z)])
+                        for (int x = bx; x != s + dir; x += dir) wall[at(x, y, z)] = 0;
+        };
+        auto extrude_y = [&](int by, int dir) {
+            int s = by + dir;
+            while (s > 0 && s < Y - 1 && !open_y(s)) s += dir;
+            if (s <= 0 || s >= Y - 1) return;
+            for (int z = 0; z < Z; ++z)
+                for (int x = 0; x < X; ++x)
+                    if (!wall[at(x, s, z)])
+                        for (int y = by; y != s + dir; y += dir) wall[at(x, y, z)] = 0;
+        };
+        if (face[0] != LBM355_FACE_WALL) extrude_x(0, 1);
+        if (face[1] != LBM355_FACE_WALL) extrude_x(X - 1, -1);
+        if (face[2] != LBM355_FACE_WALL) extrude_y(0, 1);
+        if (face[3] != LBM355_FACE_WALL) extrude_y(Y - 1, -1);
+        for (int64_t i = 0; i < N; ++i) {
+            const int l = map[i];
+            const bool w = wall[i] || l == L_WALL;
+            if (!w) {
+                if (l == L_IN_XM || l == L_OUT_XM) area[0]++;
+                if (l == L_IN_XP || l == L_OUT_XP) area[1]++;
+                if (l == L_OUT_YM) area[2]++;
+                if (l == L_OUT_YP) area[3]++;
+                if (l == L_FLUID) n_fluid++;
+            } else {
+                n_wall++;
+            }
+            if (wall[i]) map[i] = L_WALL;  // projectWall
+        }
+        for (int f = 0; f < 4; ++f)
+            if (face[f] != LBM355_FACE_WALL && area[f] <= 0) terminate = true;
+        return LBM355_OK;
+    }
+};
+
+// ---------------------------------------------------------------------------
+struct lbm355_case {
+    lbm355_case_desc d{};
+    Labels lab;
+    int storage = 1, model = 0;
+    int nbx = 0, nby = 0, nbz = 0;
+    std::vector<int32_t> brick_id;      // brick grid -> storage index or -1
+    std::vector<int32_t> brick_xyz;     // storage index -> (bx,by,bz)
+    int32_t n_active = 0, n_stored = 0, n_bc = 0;
+    std::vector<int32_t> bc_cell;       // storage brick * 64 + lane of each BC slot
+    std::vector<uint8_t> bc_label_h;
+    double inflow[2] = {0, 0};
+    int64_t steps = 0;
+    int64_t macro_valid_step = -1;      // step index whose rho,u are in the macro buffer
+    int64_t stats_steps = 0;            // steps accumulated since the last reset
+    int64_t stats_first = 0;
+    bool strain_valid = false;
+
+    hipStream_t stream = nullptr;
+    DevBuf f, stats, macro, strain, force, nbr, label, bc_base, bc_val, bc_read, bc_wmask, bc_factor, bc_label,
+        origin, step, devcase, list, bc_list, done, flux;
+    CaseDev host{};
+    int bricks_per_group = 4;
+
+    ~lbm355_case()
+    {
+        if (stream) { (void)hipStreamSynchronize(stream); (void)hipStreamDestroy(stream); }
+    }
+
+    int64_t cell_index(int x, int y, int z) const { return x + int64_t(d.nx) * (y + int64_t(d.ny) * z); }
+    int32_t stored_brick(int x, int y, int z) const
+    {
+        if (x < 0 || y < 0 || z < 0 || x >= d.nx || y >= d.ny || z >= d.nz) return -1;
+        return brick_id[(x >> 2) + int64_t(nbx) * ((y >> 2) + int64_t(nby) * (z >> 2))];
+    }
+    // element offset of slot `slot` of cell (x,y,z); the dummy brick if absent
+    int32_t slot_offset(int x, int y, int z, int slot) const
+    {
+        int32_t b = stored_brick(x, y, z);
+        int lane = 0;
+        if (b < 0) b = n_stored;
+        else lane = (x & 3) | ((y & 3) << 2) | ((z & 3) << 4);
+        return (b * kQ + slot) * kLanes + lane;
+    }
+};
+
+namespace {
+
+// the partner reads of a BC cell: the reference pulls the populations of a
+// virtual cell one plane inwards with the (clamped) indices below
+// (sim_tcpc.cu:246,256,263,269; lbm_streaming.h:10-40).  For each direction q
+// the source is cell C_q; under AA its post-collision f_q of the previous step
+// sits in slot q of C_q + c_q after an odd step and in slot opp(q) of C_q
+// after an even one.
+void partner_sources(const lbm355_case& c, int x, int y, int z, int label, int (*src)[3])
+{
+    const int X = c.d.nx, Y = c.d.ny, Z = c.d.nz;
+    const int xp = std::min(x + 1, X - 1), xm = std::max(x - 1, 0);
+    const int yp = std::min(y + 1, Y - 1), ym = std::max(y - 1, 0);
+    const int zp = std::min(z + 1, Z - 1), zm = std::max(z - 1, 0);
+    int ax[3], ay[3], az[3];  // (xm', x', xp') passed to the streaming call
+    ax[0] = xm; ax[1] = x; ax[2] = xp;
+    ay[0] = ym; ay[1] = y; ay[2] = yp;
+    az[0] = zm; az[1] = z; az[2] = zp;
+    switch (label) {
+    case L_IN_XM: case L_OUT_XM: ax[0] = x; ax[1] = xp; ax[2] = xp + 1; break;
+    case L_IN_XP: case L_OUT_XP: ax[0] = xm - 1; ax[1] = xm; ax[2] = x; break;
+    case L_OUT_YM: ay[0] = y; ay[1] = yp; ay[2] = yp + 1; break;
+    case L_OUT_YP: ay[0] = ym - 1; ay[1] = ym; ay[2] = y; break;
+    default: break;
+    }
+    for (int q = 0; q < kQ; ++q) {
+        // f_q comes from the upstream index: cx=-1 -> xp', 0 -> x', +1 -> xm'
+        src[q][0] = ax[1 - qx(q)];
+        src[q][1] = ay[1 - qy(q)];
+        src[q][2] = az[1 - qz(q)];
+    }
+}
+
+int validate(const lbm355_case_desc* d)
+{
+    if (!d) return fail(LBM355_EINVAL, "null descriptor");
+    if (d->nx < 4 || d->ny < 4 || d->nz < 4) return fail(LBM355_EINVAL, "grid must be at least 4 cells per axis");
+    if (!d->geometry) return fail(LBM355_EINVAL, "null geometry");
+    if (d->storage < 0 || d->storage > 3) return fail(LBM355_EINVAL, "unknown storage");
+    if (d->collision < 0 || d->collision > 3) return fail(LBM355_EINVAL, "unknown collision");
+    if (!(d->nu_ref > 0.0)) return fail(LBM355_EINVAL, "LBM viscosity must be positive (core.h:1044)");
+    if (d->cy_enabled && (d->cy_model < 0 || d->cy_model > 2)) return fail(LBM355_EINVAL, "unknown cy_model");
+    if (d->cy_enabled && d->cy_model == LBM355_CY_FD_FORCE)
+        return fail(LBM355_EINVAL, "cy_model fd-force is not available in this build");
+    return LBM355_OK;
+}
+
+// device bytes of a case with the given brick counts
+int64_t footprint_bytes(const lbm355_case_desc& d, int64_t n_active, int64_t n_stored, int64_t n_bc)
+{
+    const int64_t sb = storage_bytes(d.storage), cb = compute_bytes(d.storage);
+    int64_t b = (n_stored + 1) * kQ * kLanes * sb;       // populations
+    b += (n_stored + 1) * kLanes;                         // labels
+    b += n_active * (kQ * 4 + 4 + 6);                     // nbr, bc_base, origin
+    b += n_active * kMacroFields * kLanes * cb;           // macro
+    b += n_active * 6 * kLanes * cb;                      // strain
+    if (d.stats_every > 0) b += n_active * kStatFields * kLanes * 8;
+    b += n_bc * (4 * cb + 2 * kQ * 4 + 4 + 4 + 1 + 8);   // bc tables
+    b += n_active * 8;                                    // brick list
+    return b;
+}
+
+int build_index(lbm355_case& c)
+{
+    const lbm355_case_desc& d = c.d;
+    const int X = d.nx, Y = d.ny, Z = d.nz;
+    const Labels& L = c.lab;
+    c.nbx = (X + 3) / 4; c.nby = (Y + 3) / 4; c.nbz = (Z + 3) / 4;
+    const int64_t NB = int64_t(c.nbx) * c.nby * c.nbz;
+    // bricks touched by the AA update: the 3x3x3 neighbourhood of every
+    // non-wall cell, plus the partner reads of BC cells
+    std::vector<uint8_t> active_b(NB, 0), stored_b(NB, 0);
+    auto bidx = [&](int x, int y, int z) { return (x >> 2) + int64_t(c.nbx) * ((y >> 2) + int64_t(c.nby) * (z >> 2)); };
+    for (int z = 0; z < Z; ++z)
+        for (int y = 0; y < Y; ++y)
+            for (int x = 0; x < X; ++x) {
+                if (L.map[L.at(x, y, z)] == L_WALL) continue;
+                active_b[bidx(x, y, z)] = 1;
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int a = x + dx, b = y + dy, e = z + dz;
+                            if (a < 0 || b < 0 || e < 0 || a >= X || b >= Y || e >= Z) continue;
+                            stored_b[bidx(a, b, e)] = 1;
+                        }
+                const int l = L.map[L.at(x, y, z)];
+                if (is_bc_label(l)) {
+                    int src[kQ][3];
+                    partner_sources(c, x, y, z, l, src);
+                    for (int q = 0; q < kQ; ++q) {
+                        const int sx = src[q][0], sy = src[q][1], sz = src[q][2];
+                        if (sx >= 0 && sy >= 0 && sz >= 0 && sx < X && sy < Y && sz < Z) stored_b[bidx(sx, sy, sz)] = 1;
+                        const int tx = sx + qx(q), ty = sy + qy(q), tz = sz + qz(q);
+                        if (tx >= 0 && ty >= 0 && tz >= 0 && tx < X && ty < Y && tz < Z) stored_b[bidx(tx, ty, tz)] = 1;
+                    }
+                }
+            }
+    // order: active bricks by Morton code, then shell bricks by Morton code
+    std::vector<std::pair<uint64_t, int64_t>> act, shell;
+    for (int bz = 0; bz < c.nbz; ++bz)
+        for (int by = 0; by < c.nby; ++by)
+            for (int bx = 0; bx < c.nbx; ++bx) {
+                const int64_t i = bx + int64_t(c.nbx) * (by + int64_t(c.nby) * bz);
+                const uint64_t key = spread3(bx) | (spread3(by) << 1) | (spread3(bz) << 2);
+                if (active_b[i]) act.push_back({key, i});
+                else if (stored_b[i]) shell.push_back({key, i});
+            }
+    std::sort(act.begin(), act.end());
+    std::sort(shell.begin(), shell.end());
+    c.n_active = int32_t(act.size());
+    c.n_stored = int32_t(act.size() + shell.size());
+    if (int64_t(c.n_stored + 1) * kQ * kLanes >= (int64_t(1) << 31))
+        return fail(LBM355_EINVAL, "case too large for 32-bit population offsets");
+    c.brick_id.assign(NB, -1);
+    c.brick_xyz.resize(size_t(c.n_stored) * 3);
+    int32_t n = 0;
+    for (auto* v : {&act, &shell})
+        for (auto& kv : *v) {
+            const int64_t i = kv.second;
+            c.brick_id[i] = n;
+            c.brick_xyz[3 * n + 0] = int32_t(i % c.nbx);
+            c.brick_xyz[3 * n + 1] = int32_t((i / c.nbx) % c.nby);
+            c.brick_xyz[3 * n + 2] = int32_t(i / (int64_t(c.nbx) * c.nby));
+            ++n;
+        }
+    return LBM355_OK;
+}
+
+template <class T>
+int upload(DevBuf& buf, const std::vector<T>& v)
+{
+    if (buf.alloc(std::max<size_t>(v.size() * sizeof(T), 16)) != 0) return fail(LBM355_ENOMEM, "hipMalloc failed");
+    if (!v.empty()) HIP_TRY(hipMemcpy(buf.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return LBM355_OK;
+}
+
+int alloc_zero(lbm355_case& c, DevBuf& buf, size_t bytes)
+{
+    if (buf.alloc(std::max<size_t>(bytes, 16)) != 0) return fail(LBM355_ENOMEM, "hipMalloc failed");
+    LAUNCH_TRY(launch_zero(buf.p, int64_t((std::max<size_t>(bytes, 16) + 3) / 4 * 4), c.stream));
+    return LBM355_OK;
+}
+
+int setup_device(lbm355_case& c)
+{
+    const lbm355_case_desc& d = c.d;
+    const Labels& L = c.lab;
+    const int X = d.nx, Y = d.ny, Z = d.nz;
+    const int na = c.n_active, ns = c.n_stored;
+    HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+
+    // labels per stored brick (+ dummy = wall)
+    std::vector<uint8_t> lab(size_t(ns + 1) * kLanes, L_WALL);
+    for (int b = 0; b < ns; ++b)
+        for (int l = 0; l < kLanes; ++l) {
+            const int x = c.brick_xyz[3 * b] * 4 + (l & 3);
+            const int y = c.brick_xyz[3 * b + 1] * 4 + ((l >> 2) & 3);
+            const int z = c.brick_xyz[3 * b + 2] * 4 + (l >> 4);
+            lab[size_t(b) * kLanes + l] = (x < X && y < Y && z < Z) ? L.map[L.at(x, y, z)] : L_OUTSIDE;
+        }
+    // neighbour bricks of the active bricks
+    std::vector<int32_t> nbr(size_t(std::max(na, 1)) * kQ, ns);
+    std::vector<int16_t> org(size_t(std::max(na, 1)) * 3, 0);
+    for (int b = 0; b < na; ++b) {
+        const int bx = c.brick_xyz[3 * b], by = c.brick_xyz[3 * b + 1], bz = c.brick_xyz[3 * b + 2];
+        org[3 * b] = int16_t(bx); org[3 * b + 1] = int16_t(by); org[3 * b + 2] = int16_t(bz);
+        for (int o = 0; o < kQ; ++o) {
+            const int nx = bx + qx(o), ny = by + qy(o), nz = bz + qz(o);
+            int32_t s = -1;
+            if (nx >= 0 && ny >= 0 && nz >= 0 && nx < c.nbx && ny < c.nby && nz < c.nbz)
+                s = c.brick_id[nx + int64_t(c.nbx) * (ny + int64_t(c.nby) * nz)];
+            nbr[size_t(b) * kQ + o] = s < 0 ? ns : s;
+        }
+    }
+    // BC slots in (brick, lane) order
+    std::vector<int32_t> bc_base(size_t(std::max(na, 1)), -1), bc_read;
+    std::vector<uint32_t> bc_wmask;
+    std::vector<float> bc_factor;
+    c.bc_cell.clear();
+    c.bc_label_h.clear();
+    // inflow profile: centre and radius per x face
+    double cen[2][2], rad[2];
+    for (int side = 0; side < 2; ++side) {
+        const int x = side == 0 ? 0 : X - 1;
+        const uint8_t want = side == 0 ? L_IN_XM : L_IN_XP;
+        const double* cc = side == 0 ? d.ivc_center : d.svc_center;
+        double sy = 0, sz = 0;
+        int64_t cnt = 0;
+        for (int z = 0; z < Z; ++z)
+            for (int y = 0; y < Y; ++y)
+                if (L.map[L.at(x, y, z)] == want) { sy += y; sz += z; ++cnt; }
+        cen[side][0] = cc[0] < 0 && cnt ? sy / double(cnt) : cc[0];
+        cen[side][1] = cc[1] < 0 && cnt ? sz / double(cnt) : cc[1];
+        double R = side == 0 ? d.ivc_radius : d.svc_radius;
+        if (R <= 0) {
+            double r2 = 0;
+            for (int z = 0; z < Z; ++z)
+                for (int y = 0; y < Y; ++y)
+                    if (L.map[L.at(x, y, z)] == want)
+                        r2 = std::max(r2, (y - cen[side][0]) * (y - cen[side][0]) + (z - cen[side][1]) * (z - cen[side][1]));
+            R = std::sqrt(r2) + 0.5;
+        }
+        rad[side] = R;
+    }
+    for (int b = 0; b < na; ++b) {
+        bc_base[b] = int32_t(c.bc_cell.size());
+        for (int l = 0; l < kLanes; ++l) {
+            const int lb = lab[size_t(b) * kLanes + l];
+            if (!is_bc_label(lb)) continue;
+            const int x = c.brick_xyz[3 * b] * 4 + (l & 3);
+            const int y = c.brick_xyz[3 * b + 1] * 4 + ((l >> 2) & 3);
+            const int z = c.brick_xyz[3 * b + 2] * 4 + (l >> 4);
+            c.bc_cell.push_back(b * kLanes + l);
+            c.bc_label_h.push_back(uint8_t(lb));
+            int src[kQ][3];
+            partner_sources(c, x, y, z, lb, src);
+            // parity 0 (this step even, the previous one odd): slot q of C_q + c_q
+            for (int q = 0; q < kQ; ++q)
+                bc_read.push_back(c.slot_offset(src[q][0] + qx(q), src[q][1] + qy(q), src[q][2] + qz(q), q));
+            // parity 1: slot opp(q) of C_q
+            for (int q = 0; q < kQ; ++q) bc_read.push_back(c.slot_offset(src[q][0], src[q][1], src[q][2], 26 - q));
+            uint32_t wm = 0;
+            for (int q = 0; q < kQ; ++q) {
+                const int tx = x + qx(q), ty = y + qy(q), tz = z + qz(q);
+                if (tx >= 0 && ty >= 0 && tz >= 0 && tx < X && ty < Y && tz < Z) wm |= 1u << q;
+            }
+            bc_wmask.push_back(wm);
+            float fac = 1.0f;
+            if (is_inflow_label(lb)) {
+                const int side = lb == L_IN_XM ? 0 : 1;
+                if (d.profile == LBM355_PROFILE_PARABOLIC) {
+                    const double r2 = ((y - cen[side][0]) * (y - cen[side][0]) + (z - cen[side][1]) * (z - cen[side][1])) /
+                                      (rad[side] * rad[side]);
+                    fac = float(std::max(0.0, 1.0 - r2));
+                } else {
+                    const int64_t area = L.area[side];
+                    fac = area > 0 ? float(1.0 / double(area)) : 0.0f;
+                }
+            }
+            bc_factor.push_back(fac);
+        }
+    }
+    c.n_bc = int32_t(c.bc_cell.size());
+
+    const size_t sb = storage_bytes(d.storage), cb = compute_bytes(d.storage);
+    if (c.f.alloc(size_t(ns + 1) * kQ * kLanes * sb) != 0) return fail(LBM355_ENOMEM, "population buffer");
+    LAUNCH_TRY(launch_fill_populations(d.storage, c.f.p, ns + 1, c.stream));
+    int rc;
+    if ((rc = upload(c.label, lab))) return rc;
+    if ((rc = upload(c.nbr, nbr))) return rc;
+    if ((rc = upload(c.origin, org))) return rc;
+    if ((rc = upload(c.bc_base, bc_base))) return rc;
+    if ((rc = upload(c.bc_read, bc_read))) return rc;
+    if ((rc = upload(c.bc_wmask, bc_wmask))) return rc;
+    if ((rc = upload(c.bc_factor, bc_factor))) return rc;
+    if ((rc = upload(c.bc_label, c.bc_label_h))) return rc;
+    if ((rc = alloc_zero(c, c.bc_val, size_t(std::max(c.n_bc, 1)) * 4 * cb))) return rc;
+    // rho = 1, u = 0 for the initial macro state
+    if ((rc = alloc_zero(c, c.macro, size_t(std::max(na, 1)) * kMacroFields * kLanes * cb))) return rc;
+    if ((rc = alloc_zero(c, c.strain, size_t(std::max(na, 1)) * 6 * kLanes * cb))) return rc;
+    if (d.stats_every > 0 && (rc = alloc_zero(c, c.stats, size_t(std::max(na, 1)) * kStatFields * kLanes * 8))) return rc;
+    if ((rc = alloc_zero(c, c.step, 8))) return rc;
+    if ((rc = alloc_zero(c, c.done, 16))) return rc;
+    if ((rc = alloc_zero(c, c.flux, 4 * 8))) return rc;
+    std::vector<BrickRef> list(size_t(std::max(na, 1)));
+    for (int b = 0; b < na; ++b) list[b] = BrickRef{0u, uint32_t(b)};
+    if ((rc = upload(c.list, list))) return rc;
+    std::vector<uint2> bl(size_t(std::max(c.n_bc, 1)));
+    for (int s = 0; s < c.n_bc; ++s) bl[s] = make_uint2(0u, uint32_t(s));
+    if ((rc = upload(c.bc_list, bl))) return rc;
+
+    CaseDev& h = c.host;
+    std::memset(&h, 0, sizeof(h));
+    h.f = c.f.p;
+    h.stats = static_cast<double*>(d.stats_every > 0 ? c.stats.p : nullptr);
+    h.macro = c.macro.p;
+    h.strain = c.strain.p;
+    h.force = nullptr;
+    h.nbr = static_cast<const int32_t*>(c.nbr.p);
+    h.label = static_cast<const uint8_t*>(c.label.p);
+    h.bc_base = static_cast<const int32_t*>(c.bc_base.p);
+    h.bc_val = c.bc_val.p;
+    h.bc_read = static_cast<const int32_t*>(c.bc_read.p);
+    h.bc_wmask = static_cast<const uint32_t*>(c.bc_wmask.p);
+    h.bc_factor = static_cast<const float*>(c.bc_factor.p);
+    h.bc_label = static_cast<const uint8_t*>(c.bc_label.p);
+    h.origin = static_cast<const int16_t*>(c.origin.p);
+    h.step = static_cast<int64_t*>(c.step.p);
+    h.n_active = na;
+    h.n_stored = ns;
+    h.n_bc = c.n_bc;
+    h.nx = X; h.ny = Y; h.nz = Z;
+    h.nu_ref = d.nu_ref;
+    h.cy_nu0 = d.cy_nu0; h.cy_lambda = d.cy_lambda; h.cy_a = d.cy_a; h.cy_n = d.cy_n;
+    h.cy_mode = d.cy_enabled ? d.cy_model : 0;
+    h.inflow_xm = c.inflow[0];
+    h.inflow_xp = c.inflow[1];
+    h.pulse_amp = d.pulse_amp;
+    h.pulse_period = d.pulse_period;
+    h.stats_every = d.stats_every;
+    h.macro_step = -1;
+    h.macro_every = 0;
+    if (c.devcase.alloc(sizeof(CaseDev)) != 0) return fail(LBM355_ENOMEM, "case constants");
+    HIP_TRY(hipMemcpy(c.devcase.p, &h, sizeof(CaseDev), hipMemcpyHostToDevice));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    return LBM355_OK;
+}
+
+// device field of the case struct, for single-value updates
+void* case_field(lbm355_case& c, const void* host_field)
+{
+    const size_t off = static_cast<const char*>(host_field) - reinterpret_cast<const char*>(&c.host);
+    return static_cast<char*>(c.devcase.p) + off;
+}
+
+int set_case_u64(lbm355_case& c, void* host_field, uint64_t v)
+{
+    std::memcpy(host_field, &v, 8);
+    LAUNCH_TRY(launch_set_u64(case_field(c, host_field), v, c.stream));
+    return LBM355_OK;
+}
+
+int set_case_f64(lbm355_case& c, double* host_field, double v)
+{
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    return set_case_u64(c, host_field, u);
+}
+
+StepLaunch single_launch(lbm355_case& c)
+{
+    StepLaunch L{};
+    L.list = static_cast<const BrickRef*>(c.list.p);
+    L.n_list = c.n_active;
+    L.cases = static_cast<CaseDev*>(c.devcase.p);
+    L.n_cases = 1;
+    L.done = static_cast<uint32_t*>(c.done.p);
+    L.bricks_per_group = c.bricks_per_group;
+    L.bc_list = static_cast<const uint2*>(c.bc_list.p);
+    L.n_bc_list = c.n_bc;
+    return L;
+}
+
+// copy a per-active-brick device array [brick][nf][64] of type R to dense
+// double arrays; `fill` for cells without an active brick
+template <class R>
+int gather_dense(lbm355_case& c, const void* dev, int nf, int field, double fill, double* out)
+{
+    const int na = c.n_active;
+    std::vector<R> h(size_t(std::max(na, 1)) * nf * kLanes);
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    if (na > 0) HIP_TRY(hipMemcpy(h.data(), dev, h.size() * sizeof(R), hipMemcpyDeviceToHost));
+    const int X = c.d.nx, Y = c.d.ny, Z = c.d.nz;
+    const int64_t N = int64_t(X) * Y * Z;
+    std::fill(out, out + N, fill);
+    for (int b = 0; b < na; ++b)
+        for (int l = 0; l < kLanes; ++l) {
+            const int x = c.brick_xyz[3 * b] * 4 + (l & 3);
+            const int y = c.brick_xyz[3 * b + 1] * 4 + ((l >> 2) & 3);
+            const int z = c.brick_xyz[3 * b + 2] * 4 + (l >> 4);
+            if (x >= X || y >= Y || z >= Z) continue;
+            const int64_t i = c.cell_index(x, y, z);
+            if (c.lab.map[i] == L_WALL) continue;
+            out[i] = double(h[(size_t(b) * nf + field) * kLanes + l]);
+        }
+    return LBM355_OK;
+}
+
+int refresh_macro(lbm355_case& c)
+{
+    if (c.macro_valid_step == c.steps - 1 || c.steps == 0) return LBM355_OK;
+    return fail(LBM355_EINVAL, "macroscopic fields of the last step were not written");
+}
+
+// big-endian writers of the legacy VTK format (vtk_writer.hpp:1-60)
+void put_be32(std::vector<char>& buf, uint32_t v)
+{
+    buf.push_back(char(v >> 24)); buf.push_back(char(v >> 16)); buf.push_back(char(v >> 8)); buf.push_back(char(v));
+}
+void put_float(std::vector<char>& buf, float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    put_be32(buf, u);
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* lbm355_version(void) { return "lbm355 1.0 (gfx950, AA bricks, ABI 1)"; }
+const char* lbm355_last_error(void) { return g_error.c_str(); }
+
+int lbm355_device_count(int* count)
+{
+    if (!count) return fail(LBM355_EINVAL, "null count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return LBM355_OK;
+}
+
+int lbm355_case_footprint(const lbm355_case_desc* desc, int64_t* bytes)
+{
+    int rc = validate(desc);
+    if (rc) return rc;
+    if (!bytes) return fail(LBM355_EINVAL, "null bytes");
+    lbm355_case c;
+    c.d = *desc;
+    if ((rc = c.lab.build(c.d))) return rc;
+    if ((rc = build_index(c))) return rc;
+    int64_t nbc = 0;
+    for (int i = 0; i < 4; ++i) nbc += c.lab.area[i];
+    *bytes = footprint_bytes(c.d, c.n_active, c.n_stored, nbc);
+    return LBM355_OK;
+}
+
+int lbm355_case_create(const lbm355_case_desc* desc, lbm355_case** out)
+{
+    int rc = validate(desc);
+    if (rc) return rc;
+    if (!out) return fail(LBM355_EINVAL, "null out");
+    *out = nullptr;
+    std::unique_ptr<lbm355_case> c(new (std::nothrow) lbm355_case());
+    if (!c) return fail(LBM355_ENOMEM, "host allocation");
+    c->d = *desc;
+    c->d.geometry = nullptr;
+    c->storage = desc->storage;
+    c->model = desc->collision;
+    c->inflow[0] = desc->profile == LBM355_PROFILE_PARABOLIC ? desc->ivc_umax : desc->ivc_flux;
+    c->inflow[1] = desc->profile == LBM355_PROFILE_PARABOLIC ? desc->svc_umax : desc->svc_flux;
+    {
+        lbm355_case_desc tmp = *desc;
+        if ((rc = c->lab.build(tmp))) return rc;
+    }
+    if ((rc = build_index(*c))) return rc;
+    if ((rc = setup_device(*c))) return rc;
+    const bool term = c->lab.terminate;
+    *out = c.release();
+    if (term) return fail(LBM355_ETERMINATE, "zero inlet or outlet area");
+    return LBM355_OK;
+}
+
+int lbm355_case_destroy(lbm355_case* c)
+{
+    delete c;
+    return LBM355_OK;
+}
+
+int lbm355_case_info_get(const lbm355_case* c, lbm355_case_info* o)
+{
+    if (!c || !o) return fail(LBM355_EINVAL, "null argument");
+    std::memset(o, 0, sizeof(*o));
+    o->fluid = c->lab.n_fluid;
+    o->wall = c->lab.n_wall;
+    o->area_ivc = c->lab.area[0];
+    o->area_svc = c->lab.area[1];
+    o->area_out5 = c->lab.area[2];
+    o->area_out6 = c->lab.area[3];
+    o->input_label3 = c->lab.in_lab[0];
+    o->input_label4 = c->lab.in_lab[1];
+    o->input_label5 = c->lab.in_lab[2];
+    o->input_label6 = c->lab.in_lab[3];
+    o->active_cells = int64_t(c->d.nx) * c->d.ny * c->d.nz - c->lab.n_wall;
+    o->bricks_active = c->n_active;
+    o->bricks_stored = c->n_stored;
+    int64_t bytes = 0;
+    for (const DevBuf* b : {&c->f, &c->stats, &c->macro, &c->strain, &c->force, &c->nbr, &c->label, &c->bc_base,
+                            &c->bc_val, &c->bc_read, &c->bc_wmask, &c->bc_factor, &c->bc_label, &c->origin,
+                            &c->step, &c->devcase, &c->list, &c->bc_list, &c->done, &c->flux})
+        bytes += int64_t(b->bytes);
+    o->device_bytes = bytes;
+    o->steps = c->steps;
+    o->terminate = c->lab.terminate ? 1 : 0;
+    return LBM355_OK;
+}
+
+int lbm355_case_set_inflow(lbm355_case* c, double ivc, double svc)
+{
+    if (!c) return fail(LBM355_EINVAL, "null case");
+    c->inflow[0] = ivc;
+    c->inflow[1] = svc;
+    int rc;
+    if (c->host.inflow_xm != ivc && (rc = set_case_f64(*c, &c->host.inflow_xm, ivc))) return rc;
+    if (c->host.inflow_xp != svc && (rc = set_case_f64(*c, &c->host.inflow_xp, svc))) return rc;
+    return LBM355_OK;
+}
+
+int lbm355_case_step(lbm355_case* c, int64_t n)
+{
+    if (!c) return fail(LBM355_EINVAL, "null case");
+    if (n <= 0) return LBM355_OK;
+    int rc;
+    // rho,u of the last step of this call are kept for get_field / VTK
+    if ((rc = set_case_u64(*c, &c->host.macro_step, uint64_t(c->steps + n - 1)))) return rc;
+    const StepLaunch L = single_launch(*c);
+    for (int64_t s = 0; s < n; ++s) LAUNCH_TRY(launch_step(c->storage, c->model, c->host.cy_mode, L, c->stream));
+    if (c->d.stats_every > 0) {
+        const int64_t k = c->d.stats_every;
+        c->stats_steps += (c->steps + n) / k - c->steps / k;
+    }
+    c->steps += n;
+    c->macro_valid_step = c->steps - 1;
+    c->strain_valid = false;
+    return LBM355_OK;
+}
+
+int lbm355_case_sync(lbm355_case* c)
+{
+    if (!c) return fail(LBM355_EINVAL, "null case");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return LBM355_OK;
+}
+
+int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_ms)
+{
+    if (!c || !avg_ms || n <= 0) return fail(LBM355_EINVAL, "bad argument");
+    hipEvent_t a, b;
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    const StepLaunch L = single_launch(*c);
+    HIP_TRY(hipEventRecord(a, c->stream));
+    for (int64_t s = 0; s < n; ++s) LAUNCH_TRY(launch_step(c->storage, c->model, c->host.cy_mode, L, c->stream));
+    HIP_TRY(hipEventRecord(b, c->stream));
+    HIP_TRY(hipEventSynchronize(b));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (c->d.stats_every > 0) {
+        const int64_t k = c->d.stats_every;
+        c->stats_steps += (c->steps + n) / k - c->steps / k;
+    }
+    c->steps += n;
+    *avg_ms = double(ms) / double(n);
+    return LBM355_OK;
+}
+
+int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes)
+{
+    if (!c || !host) return fail(LBM355_EINVAL, "null argument");
+    const int64_t N = int64_t(c->d.nx) * c->d.ny * c->d.nz;
+    if (bytes < size_t(N) * 8) return fail(LBM355_EINVAL, "buffer too small");
+    const bool f64 = c->storage == LBM355_F64;
+    int rc;
+    switch (field) {
+    case LBM355_FIELD_RHO: case LBM355_FIELD_VX: case LBM355_FIELD_VY: case LBM355_FIELD_VZ:
+        if (c->steps == 0) {  // the initial state: rho = 1, u = 0 everywhere
+            std::fill(host, host + N, field == LBM355_FIELD_RHO ? 1.0 : 0.0);
+            return LBM355_OK;
+        }
+        if ((rc = refresh_macro(*c))) return rc;
+        return f64 ? gather_dense<double>(*c, c->macro.p, 4, field, field == 0 ? 1.0 : 0.0, host)
+                   : gather_dense<float>(*c, c->macro.p, 4, field, field == 0 ? 1.0 : 0.0, host);
+    case LBM355_FIELD_FX: case LBM355_FIELD_FY: case LBM355_FIELD_FZ:
+        std::fill(host, host + N, 0.0);
+        return LBM355_OK;
+    case LBM355_FIELD_LABEL:
+        for (int64_t i = 0; i < N; ++i) host[i] = c->lab.map[i];
+        return LBM355_OK;
+    case LBM355_FIELD_WALLMASK:
+        for (int64_t i = 0; i < N; ++i) host[i] = c->lab.wall[i];
+        return LBM355_OK;
+    default: break;
+    }
+    if (field >= LBM355_FIELD_SUM_RUX && field <= LBM355_FIELD_SUM_RRUZZ) {
+        if (c->d.stats_every <= 0) { std::fill(host, host + N, 0.0); return LBM355_OK; }
+        return gather_dense<double>(*c, c->stats.p, kStatFields, field - LBM355_FIELD_SUM_RUX, 0.0, host);
+    }
+    if (field >= LBM355_FIELD_S11 && field <= LBM355_FIELD_S33) {
+        return f64 ? gather_dense<double>(*c, c->strain.p, 6, field - LBM355_FIELD_S11, 0.0, host)
+                   : gather_dense<float>(*c, c->strain.p, 6, field - LBM355_FIELD_S11, 0.0, host);
+    }
+    return fail(LBM355_EINVAL, "unknown field id");
+}
+
+int lbm355_case_get_populations(lbm355_case* c, double* host, size_t bytes)
+{
+    if (!c || !host) return fail(LBM355_EINVAL, "null argument");
+    const int X = c->d.nx, Y = c->d.ny, Z = c->d.nz;
+    const int64_t N = int64_t(X) * Y * Z;
+    if (bytes < size_t(N) * kQ * 8) return fail(LBM355_EINVAL, "buffer too small");
+    const size_t sb = storage_bytes(c->storage);
+    const size_t ne = size_t(c->n_stored + 1) * kQ * kLanes;
+    std::vector<unsigned char> raw(ne * sb);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(raw.data(), c->f.p, raw.size(), hipMemcpyDeviceToHost));
+    auto value = [&](int32_t off, int q) -> double {
+        switch (c->storage) {
+        case LBM355_F64: { double v; std::memcpy(&v, &raw[size_t(off) * 8], 8); return v; }
+        case LBM355_F32: { float v; std::memcpy(&v, &raw[size_t(off) * 4], 4); return v; }
+        case LBM355_F16_OFFSET: {
+            uint16_t h; std::memcpy(&h, &raw[size_t(off) * 2], 2);
+            const uint32_t sgn = uint32_t(h & 0x8000) << 16;
+            int e = (h >> 10) & 0x1F;
+            uint32_t m = h & 0x3FF, bits;
+            if (e == 0) {
+                if (m == 0) bits = sgn;
+                else { e = 1; while (!(m & 0x400)) { m <<= 1; --e; } m &= 0x3FF; bits = sgn | uint32_t(e + 112) << 23 | m << 13; }
+            } else if (e == 31) bits = sgn | 0x7F800000u | m << 13;
+            else bits = sgn | uint32_t(e + 112) << 23 | m << 13;
+            float v; std::memcpy(&v, &bits, 4);
+            return double(v + float(lattice_weight(q)));
+        }
+        default: {
+            uint16_t h; std::memcpy(&h, &raw[size_t(off) * 2], 2);
+            const uint32_t bits = uint32_t(h) << 16;
+            float v; std::memcpy(&v, &bits, 4);
+            return double(v + float(lattice_weight(q)));
+        }
+        }
+    };
+    std::fill(host, host + N * kQ, 0.0);
+    const bool even_next = (c->steps & 1) == 0;  // the next step is even: the last one was odd
+    for (int z = 0; z < Z; ++z)
+        for (int y = 0; y < Y; ++y)
+            for (int x = 0; x < X; ++x) {
+                const int64_t i = c->cell_index(x, y, z);
+                if (c->lab.map[i] == L_WALL) continue;
+                for (int q = 0; q < kQ; ++q) {
+                    int32_t off;
+                    if (even_next) {
+                        const int tx = x + qx(q), ty = y + qy(q), tz = z + qz(q);
+                        if (tx < 0 || ty < 0 || tz < 0 || tx >= X || ty >= Y || tz >= Z) continue;
+                        off = c->slot_offset(tx, ty, tz, q);
+                    } else {
+                        off = c->slot_offset(x, y, z, 26 - q);
+                    }
+                    host[int64_t(q) * N + i] = value(off, q);
+                }
+            }
+    return LBM355_OK;
+}
+
+int lbm355_case_compute_strain(lbm355_case* c)
+{
+    if (!c) return fail(LBM355_EINVAL, "null case");
+    int rc;
+    if ((rc = refresh_macro(*c))) return rc;
+    LAUNCH_TRY(launch_strain(c->storage, static_cast<const CaseDev*>(c->devcase.p), c->n_active, c->stream));
+    c->strain_valid = true;
+    return LBM355_OK;
+}
+
+int lbm355_case_flux(lbm355_case* c, int mean, double out[4])
+{
+    if (!c || !out) return fail(LBM355_EINVAL, "null argument");
+    if (!mean) {
+        LAUNCH_TRY(launch_zero(c->flux.p, 32, c->stream));
+        LAUNCH_TRY(launch_flux(c->storage, static_cast<const CaseDev*>(c->devcase.p), c->n_bc,
+                               static_cast<double*>(c->flux.p), c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        HIP_TRY(hipMemcpy(out, c->flux.p, 32, hipMemcpyDeviceToHost));
+        return LBM355_OK;
+    }
+    // mean flux from the statistics sums of the BC cells (density weighted,
+    // as the mean_velocity output, sim_tcpc.cu:724-727)
+    if (c->d.stats_every <= 0) return fail(LBM355_EINVAL, "statistics are off");
+    const int na = c->n_active;
+    std::vector<double> h(size_t(std::max(na, 1)) * kStatFields * kLanes);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (na > 0) HIP_TRY(hipMemcpy(h.data(), c->stats.p, h.size() * 8, hipMemcpyDeviceToHost));
+    const double diff = std::max<double>(1.0, double(c->stats_steps));
+    for (int p = 0; p < 4; ++p) out[p] = 0.0;
+    for (int s = 0; s < c->n_bc; ++s) {
+        const int cell = c->bc_cell[s], b = cell / kLanes, l = cell % kLanes;
+        const int lab = c->bc_label_h[s];
+        int plane, comp;
+        double sign;
+        switch (lab) {
+        case L_IN_XM: plane = 0; comp = 0; sign = 1.0; break;
+        case L_OUT_XM: plane = 0; comp = 0; sign = -1.0; break;
+        case L_IN_XP: plane = 1; comp = 0; sign = -1.0; break;
+        case L_OUT_XP: plane = 1; comp = 0; sign = 1.0; break;
+        case L_OUT_YM: plane = 2; comp = 1; sign = -1.0; break;
+        default: plane = 3; comp = 1; sign = 1.0; break;
+        }
+        out[plane] += sign * h[(size_t(b) * kStatFields + comp) * kLanes + l] / diff;
+    }
+    return LBM355_OK;
+}
+
+int lbm355_case_reset_stats(lbm355_case* c)
+{
+    if (!c) return fail(LBM355_EINVAL, "null case");
+    if (c->d.stats_every > 0)
+        LAUNCH_TRY(launch_zero(c->stats.p, int64_t(c->stats.bytes), c->stream));
+    c->stats_steps = 0;
+    c->stats_first = c->steps;
+    return LBM355_OK;
+}
+
+int lbm355_case_write_vtk(lbm355_case* c, const char* path, const lbm355_output_desc* o)
+{
+    if (!c || !path || !o) return fail(LBM355_EINVAL, "null argument");
+    const int X = c->d.nx, Y = c->d.ny, Z = c->d.nz;
+    const int64_t N = int64_t(X) * Y * Z;
+    const double dl = c->d.dl, dt = c->d.dt;
+    int rc;
+    if ((rc = lbm355_case_compute_strain(c))) return rc;
+    std::vector<double> F[16];
+    const int ids[16] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
+    for (int k = 0; k < 16; ++k) {
+        F[k].resize(N);
+        if ((rc = lbm355_case_get_field(c, ids[k], F[k].data(), size_t(N) * 8))) return rc;
+    }
+    const std::vector<double>& rho = F[0];
+    const double vconv = dl / dt;
+    const double diff = std::max(1.0, double(o->stat_steps));
+    const double cs2 = vconv * vconv / 3.0;
+    const bool cy = c->d.cy_enabled != 0;
+    auto wall_at = [&](int x, int y, int z) -> bool {
+        if (x < 0 || y < 0 || z < 0 || x >= X || y >= Y || z >= Z) return false;
+        return c->lab.wall[c->cell_index(x, y, z)] != 0;
+    };
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return fail(LBM355_EINVAL, std::string("cannot open ") + path);
+    std::vector<char> buf;
+    auto flush = [&]() { if (!buf.empty()) std::fwrite(buf.data(), 1, buf.size(), fp); buf.clear(); };
+    std::fprintf(fp, "# vtk DataFile Version 3.0\nWritten using RF writer\nBINARY\n");
+    std::fprintf(fp, "DATASET RECTILINEAR_GRID\nDIMENSIONS %d %d %d\n", X, Y, Z);
+    std::fprintf(fp, "X_COORDINATES %d float\n", X);
+    for (int x = 0; x < X; ++x) put_float(buf, float(x * dl));
+    flush();
+    std::fprintf(fp, "Y_COORDINATES %d float\n", Y);
+    for (int y = 0; y < Y; ++y) put_float(buf, float(y * dl));
+    flush();
+    std::fprintf(fp, "Z_COORDINATES %d float\n", Z);
+    for (int z = 0; z < Z; ++z) put_float(buf, float(z * dl));
+    flush();
+    std::fprintf(fp, "FIELD FieldData %d\n", 2);
+    std::fprintf(fp, "TIME %d %d float\n", 1, 1);
+    put_float(buf, float(o->time));
+    flush();
+    std::fprintf(fp, "CYCLE %d %d float\n", 1, 1);
+    put_float(buf, float(o->cycle));
+    flush();
+    std::fprintf(fp, "POINT_DATA %d\n", int(N));
+    std::fprintf(fp, "SCALARS wall int 1\nLOOKUP_TABLE default\n");
+    for (int64_t i = 0; i < N; ++i) put_be32(buf, uint32_t(int32_t(c->lab.map[i])));
+    flush();
+    // StateTcpc::outputData, sim_tcpc.cu:716-784
+    auto local_nu = [&](double g) {
+        if (!cy) return c->d.nu_ref;
+        return c->d.nu_ref + (c->d.cy_nu0 - c->d.nu_ref) * std::pow(1.0 + std::pow(g * c->d.cy_lambda, c->d.cy_a),
+                                                                   (c->d.cy_n - 1.0) / c->d.cy_a);
+    };
+    auto gamma_lbm = [&](int64_t i) {
+        const double s11 = F[10][i], s12 = F[11][i], s13 = F[12][i], s22 = F[13][i], s32 = F[14][i], s33 = F[15][i];
+        return std::sqrt(2.0) * std::sqrt(s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s32 * s32));
+    };
+    const char* names[12] = {"velocity", "mean_velocity", "lbm_rho", "pressure_pa", "gamma_dot", "nu_phys_m2_s",
+                             "mu_phys_pa_s", "wss_proxy_pa", "turbulence_intensity", "rms_velocity",
+                             "strain_diag_1_s", "strain_shear_1_s"};
+    const int dofs[12] = {3, 3, 1, 1, 1, 1, 1, 1, 1, 1, 3, 3};
+    for (int a = 0; a < 12; ++a) {
+        if (dofs[a] == 1) std::fprintf(fp, "SCALARS %s float 1\nLOOKUP_TABLE default\n", names[a]);
+        else std::fprintf(fp, "VECTORS %s float\n", names[a]);
+        for (int z = 0; z < Z; ++z)
+            for (int y = 0; y < Y; ++y)
+                for (int x = 0; x < X; ++x) {
+                    const int64_t i = c->cell_index(x, y, z);
+                    const double mvx = F[4][i] * vconv / diff, mvy = F[5][i] * vconv / diff, mvz = F[6][i] * vconv / diff;
+                    double vals[3] = {0, 0, 0};
+                    switch (a) {
+                    case 0: vals[0] = F[1][i] * vconv; vals[1] = F[2][i] * vconv; vals[2] = F[3][i] * vconv; break;
+                    case 1: vals[0] = mvx; vals[1] = mvy; vals[2] = mvz; break;
+                    case 2: vals[0] = rho[i]; break;
+                    case 3: vals[0] = (rho[i] - 1.0) * o->phys_density * cs2; break;
+                    case 4: vals[0] = gamma_lbm(i) / dt; break;
+                    case 5: vals[0] = local_nu(gamma_lbm(i)) * dl * dl / dt; break;
+                    case 6: vals[0] = local_nu(gamma_lbm(i)) * dl * dl / dt * o->phys_density; break;
+                    case 7: {
+                        const bool near = wall_at(x - 1, y, z) || wall_at(x + 1, y, z) || wall_at(x, y - 1, z) ||
+                                          wall_at(x, y + 1, z) || wall_at(x, y, z - 1) || wall_at(x, y, z + 1);
+                        const double g = gamma_lbm(i);
+                        vals[0] = near ? local_nu(g) * dl * dl / dt * o->phys_density * (g / dt) : 0.0;
+                        break;
+                    }
+                    case 8: case 9: {
+                        const double mxx = F[7][i] * vconv * vconv / diff, myy = F[8][i] * vconv * vconv / diff,
+                                     mzz = F[9][i] * vconv * vconv / diff;
+                        const double rx = std::fabs(mxx - mvx * mvx), ry = std::fabs(myy - mvy * mvy),
+                                     rz = std::fabs(mzz - mvz * mvz);
+                        const double rms = std::sqrt(rx * rx + ry * ry + rz * rz);
+                        const double mv = std::sqrt(mvx * mvx + mvy * mvy + mvz * mvz);
+                        vals[0] = a == 9 ? rms : (std::fabs(mv) <= 1e-10 ? 0.0 : rms / std::fabs(mv));
+                        break;
+                    }
+                    case 10: vals[0] = F[10][i] / dt; vals[1] = F[13][i] / dt; vals[2] = F[15][i] / dt; break;
+                    case 11: vals[0] = F[11][i] / dt; vals[1] = F[12][i] / dt; vals[2] = F[14][i] / dt; break;
+                    }
+                    for (int k = 0; k < dofs[a]; ++k) put_float(buf, float(vals[k]));
+                    if (buf.size() > (1u << 20)) flush();
+                }
+        flush();
+    }
+    std::fclose(fp);
+    return LBM355_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches
+struct lbm355_batch {
+    std::vector<lbm355_case*> cases;
+    int storage = 1, model = 0;
+    int use_graph = 0;
+    int bricks_per_group = 4;
+    hipStream_t stream = nullptr;
+    DevBuf devcases, list, bc_list, done;
+    int n_list = 0, n_bc = 0;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_steps = 0;
+    ~lbm355_batch()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) { (void)hipStreamSynchronize(stream); (void)hipStreamDestroy(stream); }
+    }
+};
+
+namespace {
+StepLaunch batch_launch(lbm355_batch& b)
+{
+    StepLaunch L{};
+    L.list = static_cast<const BrickRef*>(b.list.p);
+    L.n_list = b.n_list;
+    L.cases = static_cast<CaseDev*>(b.devcases.p);
+    L.n_cases = int32_t(b.cases.size());
+    L.done = static_cast<uint32_t*>(b.done.p);
+    L.bricks_per_group = b.bricks_per_group;
+    L.bc_list = static_cast<const uint2*>(b.bc_list.p);
+    L.n_bc_list = b.n_bc;
+    return L;
+}
+
+void drop_graph(lbm355_batch& b)
+{
+    if (b.exec) (void)hipGraphExecDestroy(b.exec);
+    if (b.graph) (void)hipGraphDestroy(b.graph);
+    b.exec = nullptr;
+    b.graph = nullptr;
+    b.graph_steps = 0;
+}
+}  // namespace
+
+int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batch** out)
+{
+    if (!cases || n <= 0 || !out) return fail(LBM355_EINVAL, "bad batch arguments");
+    std::unique_ptr<lbm355_batch> b(new (std::nothrow) lbm355_batch());
+    if (!b) return fail(LBM355_ENOMEM, "host allocation");
+    b->storage = cases[0]->storage;
+    b->model = cases[0]->model;
+    b->use_graph = graph;
+    std::vector<CaseDev> cd;
+    std::vector<BrickRef> list;
+    std::vector<uint2> bl;
+    for (int i = 0; i < n; ++i) {
+        lbm355_case* c = cases[i];
+        if (!c) return fail(LBM355_EINVAL, "null case in batch");
+        if (c->storage != b->storage || c->model != b->model)
+            return fail(LBM355_EINVAL, "a batch needs one collision and one storage type");
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        b->cases.push_back(c);
+        cd.push_back(c->host);
+        for (int k = 0; k < c->n_active; ++k) list.push_back(BrickRef{uint32_t(i), uint32_t(k)});
+        for (int s = 0; s < c->n_bc; ++s) bl.push_back(make_uint2(uint32_t(i), uint32_t(s)));
+    }
+    // interleave the cases' Morton-ordered brick runs by chunks so that the
+    // XCD-contiguous split of the list gives every XCD part of every case
+    b->n_list = int(list.size());
+    b->n_bc = int(bl.size());
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    int rc;
+    if ((rc = upload(b->devcases, cd))) return rc;
+    if ((rc = upload(b->list, list))) return rc;
+    if ((rc = upload(b->bc_list, bl))) return rc;
+    if (b->done.alloc(16) != 0) return fail(LBM355_ENOMEM, "done counter");
+    HIP_TRY(hipMemset(b->done.p, 0, 16));
+    // macro output only on explicit request inside a batch
+    for (int i = 0; i < n; ++i) {
+        CaseDev* dc = static_cast<CaseDev*>(b->devcases.p) + i;
+        LAUNCH_TRY(launch_set_u64(&dc->macro_step, uint64_t(-1), b->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    *out = b.release();
+    return LBM355_OK;
+}
+
+int lbm355_batch_set_shape(lbm355_batch* b, int bpg)
+{
+    if (!b || !(bpg == 1 || bpg == 2 || bpg == 4 || bpg == 8)) return fail(LBM355_EINVAL, "bricks per group");
+    b->bricks_per_group = bpg;
+    drop_graph(*b);
+    return LBM355_OK;
+}
+
+int lbm355_batch_step(lbm355_batch* b, int64_t n)
+{
+    if (!b) return fail(LBM355_EINVAL, "null batch");
+    if (n <= 0) return LBM355_OK;
+    const StepLaunch L = batch_launch(*b);
+    int64_t done = 0;
+    if (b->use_graph) {
+        const int chunk = 16;
+        if (!b->exec) {
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            for (int s = 0; s < chunk; ++s) {
+                int r = launch_step(b->storage, b->model, 0, L, b->stream);
+                if (r) {
+                    hipGraph_t g;
+                    (void)hipStreamEndCapture(b->stream, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    return fail(LBM355_EDEVICE, kernel_error());
+                }
+            }
+            HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
+            HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->graph_steps = chunk;
+        }
+        while (n - done >= b->graph_steps) {
+            HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+            done += b->graph_steps;
+        }
+    }
+    for (; done < n; ++done) LAUNCH_TRY(launch_step(b->storage, b->model, 0, L, b->stream));
+    for (lbm355_case* c : b->cases) {
+        if (c->d.stats_every > 0) {
+            const int64_t k = c->d.stats_every;
+            c->stats_steps += (c->steps + n) / k - c->steps / k;
+        }
+        c->steps += n;
+        c->macro_valid_step = -2;  // the batch does not write rho,u
+        c->strain_valid = false;
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return LBM355_OK;
+}
+
+int lbm355_batch_autotune(lbm355_batch* b, int* best)
+{
+    if (!b) return fail(LBM355_EINVAL, "null batch");
+    (void)best;
+    return fail(LBM355_EINVAL, "autotune not available in this build");
+}
+
+int lbm355_batch_destroy(lbm355_batch* b)
+{
+    delete b;
+    return LBM355_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+// lbm355_internal.h - shared definitions of the device kernels and the host
+// driver of liblbm355.
+//
+// Data layout in HBM (per case)
+//   The grid is cut into 4x4x4 bricks; one brick = 64 cells = one wavefront.
+//   Bricks are stored only where the AA update touches memory (the non-wall
+//   cells and their 26-neighbourhood); active bricks (holding at least one
+//   non-wall cell) come first, in Morton order of the brick coordinates, then
+//   the storage-only shell bricks, then one "dummy" brick that stands in for
+//   every brick that is not stored.
+//   populations : f[brick][q][lane], lane = i + 4 j + 16 k, one 64-element
+//                 run per (brick, q): every wavefront load is one contiguous
+//                 64 x sizeof(T) segment (256 B at FP32).
+//   AA pattern  : one buffer.  An even step reads and writes the cell's own 27
+//                 slots (post-collision f_q goes to slot opp(q)); an odd step
+//                 reads slot opp(q) of the upstream neighbour and writes slot q
+//                 of the downstream neighbour.  Full-way bounce-back wall nodes
+//                 (solver/lbm3d/lbm_streaming.h:78-107) are exactly a no-op in
+//                 this scheme, so wall cells need storage but no thread.
+#pragma once
+
+#include <cstdint>
+
+namespace lbm355 {
+
+constexpr int kBrick = 4;            // cells per brick edge
+constexpr int kLanes = 64;           // cells per brick = wavefront width
+constexpr int kQ = 27;               // D3Q27
+constexpr int kStatFields = 6;       // sum rho u (3), sum (rho u)^2 (3)
+constexpr int kMacroFields = 4;      // rho, ux, uy, uz
+
+// cell labels (the reference map values, sim_tcpc.cu:194-203, plus 7/8 for
+// outflow planes on x faces) and 255 = outside the grid
+enum : uint8_t {
+    L_FLUID = 0, L_WALL = 1, L_IN_XM = 3, L_IN_XP = 4, L_OUT_YM = 5, L_OUT_YP = 6,
+    L_OUT_XP = 7, L_OUT_XM = 8, L_OUTSIDE = 255
+};
+
+__host__ __device__ constexpr bool is_bc_label(int l) { return l >= 3 && l <= 8; }
+__host__ __device__ constexpr bool is_inflow_label(int l) { return l == L_IN_XM || l == L_IN_XP; }
+
+// per-case constants and pointers read by the kernels (device resident so a
+// captured graph can be replayed while the host changes inflow or output
+// requests between replays)
+struct CaseDev {
+    void* f;                  // populations, storage type of the launch
+    double* stats;            // [active brick][6][64] statistics sums, or null
+    void* macro;              // [active brick][4][64] rho,u (compute type), or null
+    void* strain;             // [active brick][6][64] strain (compute type), or null
+    void* force;              // [active brick][3][64] body force (fd-force), or null
+    const int32_t* nbr;       // [active brick][27] storage index of neighbour bricks
+    const uint8_t* label;     // [stored brick + dummy][64]
+    const int32_t* bc_base;   // [active brick] first BC slot of the brick or -1
+    void* bc_val;             // [nbc][4] rho, ux, uy, uz (compute type)
+    const int32_t* bc_read;   // [nbc][2][27] element offsets of the partner reads
+    const uint32_t* bc_wmask; // [nbc] 27-bit mask of the odd-step writes inside the grid
+    const float* bc_factor;   // [nbc] inflow profile factor (1/area for plug)
+    const uint8_t* bc_label;  // [nbc]
+    const int16_t* origin;    // [active brick][3] brick coordinates (cells / 4)
+    int64_t* step;            // completed steps
+    int32_t n_active;         // active bricks
+    int32_t n_stored;         // stored bricks (dummy brick index = n_stored)
+    int32_t n_bc;
+    int32_t nx, ny, nz;
+    double nu_ref;            // lattice viscosity of the collision
+    double cy_nu0, cy_lambda, cy_a, cy_n;
+    double inflow_xm, inflow_xp;   // IVC / SVC scale: flux (plug) or u_max (parabolic)
+    double pulse_amp, pulse_period;
+    int32_t cy_mode;          // 0 off, 1 neq, 2 fd-force (with cy_enabled)
+    int32_t stats_every;      // 0 = off
+    int64_t macro_step;       // write rho,u at this step index (-1 = never)
+    int32_t macro_every;      // also every k steps (0 = never)
+    int32_t pad;
+};
+
+struct BrickRef {
+    uint32_t case_id;
+    uint32_t brick;
+};
+
+// launch descriptor of one fused step over a brick list
+struct StepLaunch {
+    const BrickRef* list;
+    int32_t n_list;
+    CaseDev* cases;           // device array
+    int32_t n_cases;
+    uint32_t* done;           // device counter for the last-block step increment
+    int32_t bricks_per_group; // 1, 2, 4, 8 wavefronts per workgroup
+    // BC pre-pass list
+    const uint2* bc_list;     // (case_id, bc slot)
+    int32_t n_bc_list;
+};
+
+// storage kinds (same values as LBM355_F64 ...)
+enum Storage : int { S_F64 = 0, S_F32 = 1, S_F16 = 2, S_BF16 = 3 };
+enum Model : int { M_CUM = 0, M_BGK = 1, M_MRT = 2, M_KBC = 3 };
+
+// D3Q27 lattice weights w_q, q = 9(cx+1)+3(cy+1)+(cz+1)
+__host__ __device__ constexpr double lattice_weight(int q)
+{
+    return ((q / 9 - 1) * (q / 9 - 1) + ((q / 3) % 3 - 1) * ((q / 3) % 3 - 1) + (q % 3 - 1) * (q % 3 - 1)) == 0
+               ? 8.0 / 27.0
+           : ((q / 9 - 1) * (q / 9 - 1) + ((q / 3) % 3 - 1) * ((q / 3) % 3 - 1) + (q % 3 - 1) * (q % 3 - 1)) == 1
+               ? 2.0 / 27.0
+           : ((q / 9 - 1) * (q / 9 - 1) + ((q / 3) % 3 - 1) * ((q / 3) % 3 - 1) + (q % 3 - 1) * (q % 3 - 1)) == 2
+               ? 1.0 / 54.0
+               : 1.0 / 216.0;
+}
+
+// launchers (lbm355_kernels.hip); all asynchronous on `stream`
+int launch_step(int storage, int model, int cy_mode, const StepLaunch& L, void* stream);
+int launch_fill_populations(int storage, void* f, int64_t n_bricks, void* stream);
+int launch_strain(int storage, const CaseDev* dev_case, int n_active, void* stream);
+int launch_fd_force_prepass(int storage, const CaseDev* dev_case, int n_active, void* stream);
+int launch_flux(int storage, const CaseDev* dev_case, int n_bc, double* dev_out, void* stream);
+int launch_set_u64(void* dev_ptr, uint64_t value, void* stream);
+int launch_zero(void* ptr, int64_t bytes, void* stream);
+const char* kernel_error();
+
+}  // namespace lbm355

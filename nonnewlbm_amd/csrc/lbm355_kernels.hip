@@ -1,0 +1,813 @@
+// lbm355_kernels.hip - gfx950 kernels of the D3Q27 step behind sim_tcpc.
+//
+// One wavefront = one 4x4x4 brick = 64 cells.  The fused kernel reads the 27
+// incoming populations of each non-wall cell with the AA access pattern,
+// computes rho/u, applies the collision operator (or the boundary equilibrium
+// on BC cells) and writes the 27 outgoing populations back into the same
+// buffer.  It replaces cudaLBMKernel (solver/lbm3d/core.h:598-800) with the
+// policies LBM_STREAMING, LBM_CUM/BGK/MRT/KBC_N4, LBM_BC_Tcpc and MacroTcpc
+// (solver/sim_tcpc/sim_tcpc.cu:187-418).
+//
+// HBM traffic per non-wall cell and step: 27 loads + 27 stores of the storage
+// type (the algorithmic 2*27*s bytes), 1 label byte, and on request 4 macro
+// values and 6 FP64 statistics read-modify-writes.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "lbm355_internal.h"
+
+namespace lbm355 {
+
+// ---------------------------------------------------------------------------
+// lattice helpers (q = 9(cx+1) + 3(cy+1) + (cz+1))
+__device__ __forceinline__ constexpr int qx(int q) { return q / 9 - 1; }
+__device__ __forceinline__ constexpr int qy(int q) { return (q / 3) % 3 - 1; }
+__device__ __forceinline__ constexpr int qz(int q) { return q % 3 - 1; }
+
+template <class R>
+__device__ __forceinline__ R wgt(int q)
+{
+    return R(lattice_weight(q));
+}
+
+// ---------------------------------------------------------------------------
+// storage policies: T is what lives in HBM, R the arithmetic type
+template <int S>
+struct Pop;
+
+template <>
+struct Pop<S_F64> {
+    using T = double;
+    using R = double;
+    __device__ static R ld(const T* p, int i, int) { return p[i]; }
+    __device__ static void st(T* p, int i, int, R v) { p[i] = v; }
+};
+
+template <>
+struct Pop<S_F32> {
+    using T = float;
+    using R = float;
+    __device__ static R ld(const T* p, int i, int) { return p[i]; }
+    __device__ static void st(T* p, int i, int, R v) { p[i] = v; }
+};
+
+// f_q - w_q in IEEE half: the deviation from the rest state is what needs
+// the mantissa (prior art for the offset: solver/lbm3d/lbm_eq_well.h:7-46)
+template <>
+struct Pop<S_F16> {
+    using T = _Float16;
+    using R = float;
+    __device__ static R ld(const T* p, int i, int q) { return float(p[i]) + wgt<float>(q); }
+    __device__ static void st(T* p, int i, int q, R v) { p[i] = _Float16(v - wgt<float>(q)); }
+};
+
+// f_q - w_q in bfloat16 (round to nearest even on store)
+template <>
+struct Pop<S_BF16> {
+    using T = uint16_t;
+    using R = float;
+    __device__ static R ld(const T* p, int i, int q)
+    {
+        return __uint_as_float(uint32_t(p[i]) << 16) + wgt<float>(q);
+    }
+    __device__ static void st(T* p, int i, int q, R v)
+    {
+        uint32_t u = __float_as_uint(v - wgt<float>(q));
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        p[i] = uint16_t(u >> 16);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// 1-D central-moment transforms of a velocity-ordered triple (f-, f0, f+)
+// (Geier et al. 2015, the chimera transform used by lbm_col_cum.h:50-146 and
+// its inverse, lbm_col_cum.h:319-415)
+template <class R>
+__device__ __forceinline__ void to_central(R& a, R& b, R& c, R u)
+{
+    const R s = c + a, d = c - a;
+    const R k0 = s + b;
+    const R k1 = d - u * k0;
+    const R k2 = s - R(2) * u * d + u * u * k0;
+    a = k0; b = k1; c = k2;
+}
+template <class R>
+__device__ __forceinline__ void from_central(R& a, R& b, R& c, R u)
+{
+    const R k0 = a, k1 = b, k2 = c;
+    const R fm = R(0.5) * ((u * u - u) * k0 + (R(2) * u - R(1)) * k1 + k2);
+    const R fp = fm + u * k0 + k1;
+    a = fm;
+    b = k0 - fm - fp;
+    c = fp;
+}
+
+// index of the central moment kappa_{abc} (orders along x, y, z)
+__device__ __forceinline__ constexpr int K(int a, int b, int c) { return 9 * a + 3 * b + c; }
+
+// Cumulant collision of lbm_col_cum.h (default build: omega1 = 1/(3 nu + 1/2),
+// every other rate 1, so the 3rd..6th order cumulants relax to zero and the
+// post-collision higher central moments are products of 2nd-order ones).
+// The first central moments are mirrored, which applies the force
+// (lbm_col_cum.h:313-315).
+template <class R>
+__device__ __forceinline__ void collide_cumulant(R (&m)[27], R rho, R ux, R uy, R uz, R om)
+{
+#pragma unroll
+    for (int a = 0; a < 9; ++a) to_central(m[3 * a], m[3 * a + 1], m[3 * a + 2], uz);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) to_central(m[9 * a + c], m[9 * a + 3 + c], m[9 * a + 6 + c], uy);
+#pragma unroll
+    for (int b = 0; b < 9; ++b) to_central(m[b], m[9 + b], m[18 + b], ux);
+
+    const R inv = R(1) / rho;
+    const R om1 = R(1) - om;
+    const R k000 = m[K(0, 0, 0)];
+    const R c110 = om1 * m[K(1, 1, 0)];
+    const R c101 = om1 * m[K(1, 0, 1)];
+    const R c011 = om1 * m[K(0, 1, 1)];
+    const R dxy = om1 * (m[K(2, 0, 0)] - m[K(0, 2, 0)]);
+    const R dxz = om1 * (m[K(2, 0, 0)] - m[K(0, 0, 2)]);
+    const R third = R(1) / R(3);
+    const R c200 = (dxy + dxz + k000) * third;
+    const R c020 = (dxz - R(2) * dxy + k000) * third;
+    const R c002 = (dxy - R(2) * dxz + k000) * third;
+    const R k211 = (c200 * c011 + R(2) * c101 * c110) * inv;
+    const R k121 = (c020 * c101 + R(2) * c110 * c011) * inv;
+    const R k112 = (c002 * c110 + R(2) * c011 * c101) * inv;
+    const R k220 = (c020 * c200 + R(2) * c110 * c110) * inv;
+    const R k022 = (c002 * c020 + R(2) * c011 * c011) * inv;
+    const R k202 = (c200 * c002 + R(2) * c101 * c101) * inv;
+    const R k222 = (c200 * k022 + c020 * k202 + c002 * k220 + R(4) * (c011 * k211 + c101 * k121 + c110 * k112)) * inv -
+                   (R(16) * c110 * c101 * c011 + R(4) * (c101 * c101 * c020 + c011 * c011 * c200 + c110 * c110 * c002) +
+                    R(2) * c200 * c020 * c002) * (inv * inv);
+
+    m[K(1, 0, 0)] = -m[K(1, 0, 0)];
+    m[K(0, 1, 0)] = -m[K(0, 1, 0)];
+    m[K(0, 0, 1)] = -m[K(0, 0, 1)];
+    m[K(1, 1, 0)] = c110; m[K(1, 0, 1)] = c101; m[K(0, 1, 1)] = c011;
+    m[K(2, 0, 0)] = c200; m[K(0, 2, 0)] = c020; m[K(0, 0, 2)] = c002;
+    m[K(1, 1, 1)] = R(0);
+    m[K(2, 1, 0)] = R(0); m[K(2, 0, 1)] = R(0); m[K(1, 2, 0)] = R(0);
+    m[K(0, 2, 1)] = R(0); m[K(1, 0, 2)] = R(0); m[K(0, 1, 2)] = R(0);
+    m[K(2, 1, 1)] = k211; m[K(1, 2, 1)] = k121; m[K(1, 1, 2)] = k112;
+    m[K(2, 2, 0)] = k220; m[K(0, 2, 2)] = k022; m[K(2, 0, 2)] = k202;
+    m[K(2, 2, 1)] = R(0); m[K(2, 1, 2)] = R(0); m[K(1, 2, 2)] = R(0);
+    m[K(2, 2, 2)] = k222;
+
+#pragma unroll
+    for (int b = 0; b < 9; ++b) from_central(m[b], m[9 + b], m[18 + b], ux);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) from_central(m[9 * a + c], m[9 * a + 3 + c], m[9 * a + 6 + c], uy);
+#pragma unroll
+    for (int a = 0; a < 9; ++a) from_central(m[3 * a], m[3 * a + 1], m[3 * a + 2], uz);
+}
+
+// 1-D factors of the product-form equilibrium: phi(0) = 2/3 - u^2,
+// phi(+-1) = (u^2 +- u + 1/3)/2 (lbm_col_bgk.h:43-53, lbm_eq_inv_cum.h:19-49)
+template <class R>
+__device__ __forceinline__ void phi3(R u, R (&p)[3])
+{
+    const R uu = u * u;
+    p[1] = R(2) / R(3) - uu;
+    p[0] = R(0.5) * (uu - u + R(1) / R(3));
+    p[2] = R(0.5) * (uu + u + R(1) / R(3));
+}
+
+// boundary equilibrium f_eq^INV_CUM(rho, u) including the extra rho of feq_pmm
+// (lbm_eq_inv_cum.h:49), which matters at inflow cells (rho != 1)
+template <class R>
+__device__ __forceinline__ void feq_boundary(R (&f)[27], R rho, R ux, R uy, R uz)
+{
+    R px[3], py[3], pz[3];
+    phi3(ux, px); phi3(uy, py); phi3(uz, pz);
+#pragma unroll
+    for (int q = 0; q < 27; ++q) f[q] = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
+    f[18] *= rho;
+}
+
+// improved BGK with product-form equilibrium and source term
+// (lbm_col_bgk.h:14-140)
+template <class R>
+__device__ __forceinline__ void collide_bgk(R (&f)[27], R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz, R om)
+{
+    R px[3], py[3], pz[3];
+    phi3(ux, px); phi3(uy, py); phi3(uz, pz);
+    const R src = (R(1) - R(0.5) * om) * R(3) / rho;
+    const R uF = ux * Fx + uy * Fy + uz * Fz;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const R feq = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
+        const R cF = R(qx(q)) * Fx + R(qy(q)) * Fy + R(qz(q)) * Fz;
+        f[q] += (feq - f[q]) * om + src * (cF - uF) * feq;
+    }
+}
+
+// regularised collision with a Smagorinsky rate, "MRT_LES"
+// (lbm_col_mrt.h:14-77); the force is not applied, as in the reference
+template <class R>
+__device__ __forceinline__ void collide_mrt(R (&f)[27], R rho, R ux, R uy, R uz, R nu)
+{
+    R Pxx = 0, Pyy = 0, Pzz = 0, Pxy = 0, Pxz = 0, Pyz = 0;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const R cx = R(qx(q)), cy = R(qy(q)), cz = R(qz(q));
+        Pxx += cx * cx * f[q]; Pyy += cy * cy * f[q]; Pzz += cz * cz * f[q];
+        Pxy += cx * cy * f[q]; Pxz += cx * cz * f[q]; Pyz += cy * cz * f[q];
+    }
+    const R third = R(1) / R(3);
+    const R nxx = Pxx - rho * (third + ux * ux), nyy = Pyy - rho * (third + uy * uy), nzz = Pzz - rho * (third + uz * uz);
+    const R nxy = Pxy - rho * ux * uy, nxz = Pxz - rho * ux * uz, nyz = Pyz - rho * uy * uz;
+    const R Q = R(2) * (nxx * nxx + nyy * nyy + nzz * nzz + R(2) * (nxy * nxy + nxz * nxz + nyz * nyz));
+    const R tau = R(3) * nu + R(0.5);
+    const R om = R(2) / (sqrt(tau * tau + R(18) * R(0.0342) * sqrt(Q) / rho) + tau);
+    Pxx -= om * nxx; Pyy -= om * nyy; Pzz -= om * nzz;
+    Pxy -= om * nxy; Pxz -= om * nxz; Pyz -= om * nyz;
+    const R tr = Pxx + Pyy + Pzz;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const int cx = qx(q), cy = qy(q), cz = qz(q);
+        const int c2 = cx * cx + cy * cy + cz * cz;
+        const R cPc = R(cx * cx) * Pxx + R(cy * cy) * Pyy + R(cz * cz) * Pzz +
+                      R(2) * (R(cx * cy) * Pxy + R(cx * cz) * Pxz + R(cy * cz) * Pyz);
+        const R cu = R(cx) * ux + R(cy) * uy + R(cz) * uz;
+        f[q] = wgt<R>(q) * (rho * (R(2.5) - R(1.5) * R(c2) + R(3) * cu) + R(4.5) * cPc - R(1.5) * tr);
+    }
+}
+
+// shear part s_q(D + T + Q) of KBC N4 from raw moments
+// (lbm_col_kbc_n.h:21-219): M2 = {xx,yy,zz,xy,xz,yz}, M3 = {xyy,xzz,xxy,yzz,xxz,yyz,xyz}
+template <class R>
+__device__ __forceinline__ R kbc_shear(int q, const R (&M2)[6], const R (&M3)[7])
+{
+    const int cx = qx(q), cy = qy(q), cz = qz(q);
+    const int nz = (cx != 0) + (cy != 0) + (cz != 0);
+    const R Nxz = M2[0] - M2[2], Nyz = M2[1] - M2[2], Tr = M2[0] + M2[1] + M2[2];
+    if (nz == 0) return -Tr;
+    if (nz == 1) {
+        if (cx) return (R(2) * Nxz - Nyz) / R(6) + Tr / R(6) - R(cx) * (M3[0] + M3[1]) * R(0.5);
+        if (cy) return (R(2) * Nyz - Nxz) / R(6) + Tr / R(6) - R(cy) * (M3[2] + M3[3]) * R(0.5);
+        return -(Nxz + Nyz) / R(6) + Tr / R(6) - R(cz) * (M3[4] + M3[5]) * R(0.5);
+    }
+    if (nz == 2) {
+        if (!cz) return (R(cx * cy) * M2[3] + R(cx) * M3[0] + R(cy) * M3[2]) * R(0.25);
+        if (!cy) return (R(cx * cz) * M2[4] + R(cx) * M3[1] + R(cz) * M3[4]) * R(0.25);
+        return (R(cy * cz) * M2[5] + R(cy) * M3[3] + R(cz) * M3[5]) * R(0.25);
+    }
+    return R(cx * cy * cz) * M3[6] * R(0.125);
+}
+
+// entropic KBC N4 (lbm_col_kbc_n.h:1024-1259): f = k + s + h, shear part
+// relaxed with beta = 1/(6 nu + 1), higher part with the entropic gamma
+template <class R>
+__device__ __forceinline__ void collide_kbc(R (&f)[27], R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz, R nu)
+{
+    R px[3], py[3], pz[3];
+    phi3(ux, px); phi3(uy, py); phi3(uz, pz);
+    R M2[6] = {0, 0, 0, 0, 0, 0};
+    R M3[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const R cx = R(qx(q)), cy = R(qy(q)), cz = R(qz(q));
+        const R v = f[q];
+        M2[0] += cx * cx * v; M2[1] += cy * cy * v; M2[2] += cz * cz * v;
+        M2[3] += cx * cy * v; M2[4] += cx * cz * v; M2[5] += cy * cz * v;
+        M3[0] += cx * cy * cy * v; M3[1] += cx * cz * cz * v; M3[2] += cx * cx * cy * v;
+        M3[3] += cy * cz * cz * v; M3[4] += cx * cx * cz * v; M3[5] += cy * cy * cz * v;
+        M3[6] += cx * cy * cz * v;
+    }
+    const R third = R(1) / R(3);
+    // non-equilibrium parts of the moments: s is linear in them
+    M2[0] -= rho * (third + ux * ux); M2[1] -= rho * (third + uy * uy); M2[2] -= rho * (third + uz * uz);
+    M2[3] -= rho * ux * uy; M2[4] -= rho * ux * uz; M2[5] -= rho * uy * uz;
+    M3[0] -= rho * ux * (third + uy * uy); M3[1] -= rho * ux * (third + uz * uz);
+    M3[2] -= rho * uy * (third + ux * ux); M3[3] -= rho * uy * (third + uz * uz);
+    M3[4] -= rho * uz * (third + ux * ux); M3[5] -= rho * uz * (third + uy * uy);
+    M3[6] -= rho * ux * uy * uz;
+    R num = 0, den = 0;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const R feq = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
+        const R ds = kbc_shear<R>(q, M2, M3);
+        const R dh = f[q] - feq - ds;
+        const R ifeq = R(1) / feq;
+        num += ds * dh * ifeq;
+        den += dh * dh * ifeq;
+    }
+    const R beta = R(1) / (R(6) * nu + R(1));
+    const R ib = R(1) / beta;
+    // 0/0 only when dh == 0 everywhere, where any finite gamma gives the same f
+    const R gamma = den != R(0) ? ib - (R(2) - ib) * num / den : R(2);
+    const R src = (R(1) - beta) * R(3) / rho;
+    const R uF = ux * Fx + uy * Fy + uz * Fz;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const R feq = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
+        const R ds = kbc_shear<R>(q, M2, M3);
+        const R dh = f[q] - feq - ds;
+        const R cF = R(qx(q)) * Fx + R(qy(q)) * Fy + R(qz(q)) * Fz;
+        f[q] -= beta * (R(2) * ds + gamma * dh) - src * (cF - uF) * feq;
+    }
+}
+
+// Carreau-Yasuda: nu = nu_ref + (nu0 - nu_ref) (1 + (lambda gamma)^a)^((n-1)/a)
+// (solver/lbm3d/core.h:766, sim_tcpc.cu:697-707)
+__device__ __forceinline__ double cy_pow(double b, double e) { return pow(b, e); }
+__device__ __forceinline__ float cy_pow(float b, float e) { return powf(b, e); }
+
+// local viscosity of the "neq" coupling: gamma from the deviatoric
+// non-equilibrium second moment P, gamma * tau(gamma) = 3 sqrt(2)|P|/(2 rho),
+// solved by 8 safeguarded Newton steps (the same iteration as the oracle)
+template <class R>
+__device__ __forceinline__ R neq_viscosity(const R (&f)[27], R rho, R ux, R uy, R uz, const CaseDev& C)
+{
+    R Pxx = 0, Pyy = 0, Pzz = 0, Pxy = 0, Pxz = 0, Pyz = 0;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const R cx = R(qx(q)), cy = R(qy(q)), cz = R(qz(q));
+        Pxx += cx * cx * f[q]; Pyy += cy * cy * f[q]; Pzz += cz * cz * f[q];
+        Pxy += cx * cy * f[q]; Pxz += cx * cz * f[q]; Pyz += cy * cz * f[q];
+    }
+    Pxx -= rho * ux * ux; Pyy -= rho * uy * uy; Pzz -= rho * uz * uz;
+    Pxy -= rho * ux * uy; Pxz -= rho * ux * uz; Pyz -= rho * uy * uz;
+    const R tr = (Pxx + Pyy + Pzz) / R(3);
+    const R dxx = Pxx - tr, dyy = Pyy - tr, dzz = Pzz - tr;
+    const R s2 = dxx * dxx + dyy * dyy + dzz * dzz + R(2) * (Pxy * Pxy + Pxz * Pxz + Pyz * Pyz);
+    const R G = R(3) * sqrt(R(2)) * sqrt(s2) / (R(2) * rho);
+    const R nr = R(C.nu_ref), n0 = R(C.cy_nu0), lam = R(C.cy_lambda), a = R(C.cy_a), n = R(C.cy_n);
+    const R e = (n - R(1)) / a;
+    const R tr_ = R(3) * nr + R(0.5), t0 = R(3) * n0 + R(0.5);
+    R lo = G / fmax(tr_, t0), hi = G / fmin(tr_, t0);
+    R g = hi;
+#pragma unroll 1
+    for (int it = 0; it < 8; ++it) {
+        const R x = cy_pow(lam * g, a);
+        const R b = R(1) + x;
+        const R pw = cy_pow(b, e);
+        const R nu = nr + (n0 - nr) * pw;
+        const R h = g * (R(3) * nu + R(0.5)) - G;
+        const R dgnu = nr + (n0 - nr) * pw * (R(1) + (n - R(1)) * x / b);
+        const R dh = R(3) * dgnu + R(0.5);
+        if (h > R(0)) hi = g; else lo = g;
+        R gn = g - h / dh;
+        if (!(gn > lo && gn < hi)) gn = R(0.5) * (lo + hi);
+        g = gn;
+    }
+    return nr + (n0 - nr) * cy_pow(R(1) + cy_pow(lam * g, a), e);
+}
+
+// ---------------------------------------------------------------------------
+// neighbour brick of the wavefront for a brick offset in {-1,0,1}^3 selected
+// per lane: t[] holds the 27 storage indices (wave-uniform, in SGPRs), the
+// booleans say whether the lane's shifted cell left the brick along x/y/z
+// in the direction (dx,dy,dz).  Compile-time offsets fold the select tree.
+__device__ __forceinline__ int pick_brick(const int (&t)[27], int dx, int dy, int dz, bool ex, bool ey, bool ez)
+{
+    const int a000 = t[13], a100 = t[13 + 9 * dx], a010 = t[13 + 3 * dy], a001 = t[13 + dz];
+    const int a110 = t[13 + 9 * dx + 3 * dy], a101 = t[13 + 9 * dx + dz], a011 = t[13 + 3 * dy + dz];
+    const int a111 = t[13 + 9 * dx + 3 * dy + dz];
+    const int r00 = ex ? a100 : a000, r10 = ex ? a110 : a010;
+    const int r01 = ex ? a101 : a001, r11 = ex ? a111 : a011;
+    const int r0 = ey ? r10 : r00, r1 = ey ? r11 : r01;
+    return ez ? r1 : r0;
+}
+
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return (uint64_t(1) << lane) - 1; }
+
+// ---------------------------------------------------------------------------
+// the fused step
+template <int S, int MODEL>
+__device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef br, int lane)
+{
+    using P = Pop<S>;
+    using T = typename P::T;
+    using R = typename P::R;
+    const CaseDev& C = L.cases[br.case_id];
+    const int b = int(br.brick);
+    T* __restrict__ F = static_cast<T*>(C.f);
+    const int64_t step = *C.step;
+    const int parity = int(step & 1);
+    const int lab = C.label[b * kLanes + lane];
+    const bool fluid = lab == L_FLUID;
+    const bool bc = is_bc_label(lab);
+    const uint64_t bc_lanes = __ballot(bc);
+
+    int t[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) t[q] = C.nbr[b * 27 + q];
+
+    const int i = lane & 3, j = (lane >> 2) & 3, k = lane >> 4;
+
+    R f[27];
+    R rho = R(1), ux = R(0), uy = R(0), uz = R(0);
+    uint32_t wmask = 0x7FFFFFFu;
+    if (fluid) {
+        if (parity == 0) {
+#pragma unroll
+            for (int q = 0; q < 27; ++q) f[q] = P::ld(F, (b * 27 + q) * kLanes + lane, q);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                const int cx = qx(q), cy = qy(q), cz = qz(q);
+                const int si = i - cx, sj = j - cy, sk = k - cz;
+                const int nb = pick_brick(t, -cx, -cy, -cz, (si & ~3) != 0, (sj & ~3) != 0, (sk & ~3) != 0);
+                const int l2 = (si & 3) | ((sj & 3) << 2) | ((sk & 3) << 4);
+                f[q] = P::ld(F, (nb * 27 + (26 - q)) * kLanes + l2, q);
+            }
+        }
+        R Fx = R(0), Fy = R(0), Fz = R(0);
+        if (C.force) {
+            const R* fo = static_cast<const R*>(C.force);
+            Fx = fo[(b * 3 + 0) * kLanes + lane];
+            Fy = fo[(b * 3 + 1) * kLanes + lane];
+            Fz = fo[(b * 3 + 2) * kLanes + lane];
+        }
+        // density in opposite pairs, then momentum (lbm_common.h:16-39)
+        R s = f[13];
+#pragma unroll
+        for (int q = 0; q < 13; ++q) s += f[q] + f[26 - q];
+        rho = s;
+        R mx = R(0), my = R(0), mz = R(0);
+#pragma unroll
+        for (int q = 0; q < 13; ++q) {
+            const R d = f[26 - q] - f[q];  // direction 26-q has c = -c_q
+            mx -= R(qx(q)) * d; my -= R(qy(q)) * d; mz -= R(qz(q)) * d;
+        }
+        const R ir = R(1) / rho;
+        ux = (mx + R(0.5) * Fx) * ir;
+        uy = (my + R(0.5) * Fy) * ir;
+        uz = (mz + R(0.5) * Fz) * ir;
+        R nu = R(C.nu_ref);
+        if (C.cy_mode == 1) nu = neq_viscosity<R>(f, rho, ux, uy, uz, C);
+        const R om = R(1) / (R(3) * nu + R(0.5));
+        if constexpr (MODEL == M_CUM) collide_cumulant<R>(f, rho, ux, uy, uz, om);
+        else if constexpr (MODEL == M_BGK) collide_bgk<R>(f, rho, ux, uy, uz, Fx, Fy, Fz, om);
+        else if constexpr (MODEL == M_MRT) collide_mrt<R>(f, rho, ux, uy, uz, nu);
+        else collide_kbc<R>(f, rho, ux, uy, uz, Fx, Fy, Fz, nu);
+    } else if (bc) {
+        // values prepared by the BC pre-pass for this step
+        const int slot = C.bc_base[b] + __popcll(bc_lanes & lanes_below(lane));
+        const R* bv = static_cast<const R*>(C.bc_val) + 4 * slot;
+        rho = bv[0]; ux = bv[1]; uy = bv[2]; uz = bv[3];
+        feq_boundary<R>(f, rho, ux, uy, uz);
+        wmask = C.bc_wmask[slot];
+    }
+    if (fluid || bc) {
+        if (parity == 0) {
+#pragma unroll
+            for (int q = 0; q < 27; ++q) P::st(F, (b * 27 + (26 - q)) * kLanes + lane, q, f[q]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                const int cx = qx(q), cy = qy(q), cz = qz(q);
+                const int di = i + cx, dj = j + cy, dk = k + cz;
+                const int nb = pick_brick(t, cx, cy, cz, (di & ~3) != 0, (dj & ~3) != 0, (dk & ~3) != 0);
+                const int l2 = (di & 3) | ((dj & 3) << 2) | ((dk & 3) << 4);
+                if ((wmask >> q) & 1u) P::st(F, (nb * 27 + q) * kLanes + l2, q, f[q]);
+            }
+        }
+        // macroscopic output (MacroTcpc::outputMacro, sim_tcpc.cu:337-354)
+        if (C.macro && (step == C.macro_step || (C.macro_every > 0 && (step + 1) % C.macro_every == 0))) {
+            R* mo = static_cast<R*>(C.macro);
+            mo[(b * 4 + 0) * kLanes + lane] = rho;
+            mo[(b * 4 + 1) * kLanes + lane] = ux;
+            mo[(b * 4 + 2) * kLanes + lane] = uy;
+            mo[(b * 4 + 3) * kLanes + lane] = uz;
+        }
+        if (C.stats && C.stats_every > 0 && (step + 1) % C.stats_every == 0) {
+            double* st = C.stats + (size_t)b * kStatFields * kLanes + lane;
+            const double rx = double(rho) * double(ux), ry = double(rho) * double(uy), rz = double(rho) * double(uz);
+            st[0 * kLanes] += rx;
+            st[1 * kLanes] += ry;
+            st[2 * kLanes] += rz;
+            st[3 * kLanes] += rx * rx;
+            st[4 * kLanes] += ry * ry;
+            st[5 * kLanes] += rz * rz;
+        }
+    }
+}
+
+// XCD-aware mapping: workgroups go round-robin to the 8 XCDs; remap them so
+// that each XCD walks one contiguous (Morton-ordered, hence spatially compact)
+// chunk of the brick list and its L2 sees the neighbour bricks.
+__device__ __forceinline__ int xcd_block(int bid, int grid)
+{
+    const int per = grid >> 3;
+    return (bid & 7) * per + (bid >> 3);
+}
+
+// every case of the launch has finished this step once the last workgroup is
+// done: that workgroup advances the step counters (no waiting on other
+// workgroups, so no hang is possible)
+__device__ __forceinline__ void finish_step(const StepLaunch& L)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned prev = atomicAdd(L.done, 1u);
+        if (prev == gridDim.x - 1) {
+            for (int c = 0; c < L.n_cases; ++c) *L.cases[c].step += 1;
+            *L.done = 0u;
+            __threadfence();
+        }
+    }
+}
+
+template <int S, int MODEL>
+__global__ void __launch_bounds__(512) k_step(StepLaunch L)
+{
+    const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const int blk = xcd_block(int(blockIdx.x), int(gridDim.x));
+    const int w = blk * L.bricks_per_group + wid;
+    if (w < L.n_list) {
+        const BrickRef br = L.list[w];
+        step_brick<S, MODEL>(L, br, int(threadIdx.x) & 63);
+    }
+    finish_step(L);
+}
+
+// BC pre-pass: rho (inflow) or u (outflow) from the populations that enter
+// the partner cell one plane inwards (sim_tcpc.cu:242-273).  A separate launch
+// because, under AA, the partner rewrites exactly those slots in the step.
+template <int S>
+__global__ void __launch_bounds__(256) k_bc_prepass(StepLaunch L)
+{
+    using P = Pop<S>;
+    using T = typename P::T;
+    using R = typename P::R;
+    const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= L.n_bc_list) return;
+    const uint2 e = L.bc_list[t];
+    const CaseDev& C = L.cases[e.x];
+    const int s = int(e.y);
+    const int64_t step = *C.step;
+    const int parity = int(step & 1);
+    const T* F = static_cast<const T*>(C.f);
+    const int32_t* rd = C.bc_read + (s * 2 + parity) * 27;
+    R g[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) g[q] = P::ld(F, rd[q], q);
+    R rho = g[13];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) rho += g[q] + g[26 - q];
+    const int lab = C.bc_label[s];
+    R out[4];
+    if (is_inflow_label(lab)) {
+        R pulse = R(1);
+        if (C.pulse_amp != 0.0 && C.pulse_period > 0.0)
+            pulse = R(1.0 + C.pulse_amp * sin(6.283185307179586 * double(step) / C.pulse_period));
+        const R u = (lab == L_IN_XM ? R(C.inflow_xm) : -R(C.inflow_xp)) * R(C.bc_factor[s]) * pulse;
+        out[0] = rho; out[1] = u; out[2] = R(0); out[3] = R(0);
+    } else {
+        R mx = R(0), my = R(0), mz = R(0);
+#pragma unroll
+        for (int q = 0; q < 13; ++q) {
+            const R d = g[26 - q] - g[q];
+            mx -= R(qx(q)) * d; my -= R(qy(q)) * d; mz -= R(qz(q)) * d;
+        }
+        out[0] = R(1); out[1] = mx / rho; out[2] = my / rho; out[3] = mz / rho;
+    }
+    R* bv = static_cast<R*>(C.bc_val) + 4 * s;
+    bv[0] = out[0]; bv[1] = out[1]; bv[2] = out[2]; bv[3] = out[3];
+}
+
+template <int S>
+__global__ void k_fill_rest(typename Pop<S>::T* f, int64_t n)
+{
+    using P = Pop<S>;
+    using R = typename P::R;
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int q = int((i / kLanes) % kQ);
+    // rest state f = f_eq(1, 0) = w_q (resetLattice, sim_tcpc.cu:647-655)
+    P::st(f + (i / kLanes) * kLanes, int(i % kLanes), q, R(lattice_weight(q)));
+}
+
+__global__ void k_zero(uint32_t* p, int64_t n)
+{
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// single-value device updates (kernel arguments are copied at launch, so the
+// host value can change right after the call; no memset/memcpy nodes)
+__global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
+
+// ---------------------------------------------------------------------------
+// strain S of the velocity field with the reference stencil
+// (cudaLBMKernelStress, solver/lbm3d/core.h:407-576): per axis, central
+// difference when both face neighbours are fluid, one-sided when one is, zero
+// when none is; S = 0 on non-fluid cells.  The brick and its one-cell halo are
+// staged in LDS (6x6x6 cells x (fluid flag, u)).
+template <class R>
+__global__ void __launch_bounds__(64) k_strain(const CaseDev* Cp)
+{
+    const CaseDev& C = *Cp;
+    const int b = int(blockIdx.x);
+    const int lane = int(threadIdx.x);
+    __shared__ R su[3][216];
+    __shared__ uint8_t sf[216];
+    const R* mo = static_cast<const R*>(C.macro);
+    for (int c = lane; c < 216; c += 64) {
+        const int hx = c % 6, hy = (c / 6) % 6, hz = c / 36;
+        const int gx = hx - 1, gy = hy - 1, gz = hz - 1;  // cell offset inside the brick (-1..4)
+        const int ox = (gx < 0) ? -1 : (gx > 3 ? 1 : 0);
+        const int oy = (gy < 0) ? -1 : (gy > 3 ? 1 : 0);
+        const int oz = (gz < 0) ? -1 : (gz > 3 ? 1 : 0);
+        const int nb = C.nbr[b * 27 + 13 + 9 * ox + 3 * oy + oz];
+        const int l = (gx & 3) | ((gy & 3) << 2) | ((gz & 3) << 4);
+        const bool fl = C.label[nb * kLanes + l] == L_FLUID;
+        sf[c] = fl ? 1 : 0;
+        R v0 = R(0), v1 = R(0), v2 = R(0);
+        if (fl && nb < C.n_active) {
+            v0 = mo[(nb * 4 + 1) * kLanes + l];
+            v1 = mo[(nb * 4 + 2) * kLanes + l];
+            v2 = mo[(nb * 4 + 3) * kLanes + l];
+        }
+        su[0][c] = v0; su[1][c] = v1; su[2][c] = v2;
+    }
+    __syncthreads();
+    const int i = lane & 3, j = (lane >> 2) & 3, k = lane >> 4;
+    const int c0 = (i + 1) + 6 * ((j + 1) + 6 * (k + 1));
+    R S11 = 0, S12 = 0, S13 = 0, S22 = 0, S32 = 0, S33 = 0;
+    if (sf[c0]) {
+        const int st[3] = {1, 6, 36};
+        R d[3][3];  // d[axis][component]
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int cm = c0 - st[a], cp = c0 + st[a];
+            const bool m = sf[cm], p = sf[cp];
+#pragma unroll
+            for (int comp = 0; comp < 3; ++comp) {
+                R v;
+                if (!m && !p) v = R(0);
+                else if (!m) v = su[comp][cp] - su[comp][c0];
+                else if (!p) v = su[comp][c0] - su[comp][cm];
+                else v = R(0.5) * (su[comp][cp] - su[comp][cm]);
+                d[a][comp] = v;
+            }
+        }
+        S11 = d[0][0]; S22 = d[1][1]; S33 = d[2][2];
+        S12 = R(0.5) * d[0][1] + R(0.5) * d[1][0];
+        S13 = R(0.5) * d[0][2] + R(0.5) * d[2][0];
+        S32 = R(0.5) * d[1][2] + R(0.5) * d[2][1];
+    }
+    R* so = static_cast<R*>(C.strain);
+    so[(b * 6 + 0) * kLanes + lane] = S11;
+    so[(b * 6 + 1) * kLanes + lane] = S12;
+    so[(b * 6 + 2) * kLanes + lane] = S13;
+    so[(b * 6 + 3) * kLanes + lane] = S22;
+    so[(b * 6 + 4) * kLanes + lane] = S32;
+    so[(b * 6 + 5) * kLanes + lane] = S33;
+}
+
+// ---------------------------------------------------------------------------
+// plane fluxes: out[0..3] += sum over BC cells of u_n * sign for the IVC,
+// SVC/x+ plane, label-5 and label-6 planes (compute_flows.py:107-120, with
+// inlets reported as positive inflow, compute_flows.py:133-137); wavefront
+// shuffle reduction then one FP64 atomic per wavefront and plane.
+template <class R>
+__global__ void __launch_bounds__(256) k_flux(const CaseDev* Cp, double* out)
+{
+    const CaseDev& C = *Cp;
+    const int s = int(blockIdx.x * blockDim.x + threadIdx.x);
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (s < C.n_bc) {
+        const int lab = C.bc_label[s];
+        int comp = 1, plane = 0;
+        double sign = 1.0;
+        switch (lab) {
+        case L_IN_XM: plane = 0; comp = 1; sign = 1.0; break;   // inflow along +x
+        case L_OUT_XM: plane = 0; comp = 1; sign = -1.0; break;
+        case L_IN_XP: plane = 1; comp = 1; sign = -1.0; break;  // inflow along -x
+        case L_OUT_XP: plane = 1; comp = 1; sign = 1.0; break;
+        case L_OUT_YM: plane = 2; comp = 2; sign = -1.0; break; // outward = -y
+        default: plane = 3; comp = 2; sign = 1.0; break;        // outward = +y
+        }
+        const double u = double(static_cast<const R*>(C.bc_val)[4 * s + comp]);
+        v[plane] = u * sign;
+    }
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        double x = v[p];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if ((threadIdx.x & 63) == 0 && x != 0.0) atomicAdd(out + p, x);
+    }
+}
+
+// ---------------------------------------------------------------------------
+static thread_local const char* g_kerr = "";
+const char* kernel_error() { return g_kerr; }
+
+static int check_launch()
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_kerr = hipGetErrorString(e);
+        return -3;
+    }
+    return 0;
+}
+
+template <int S, int MODEL>
+static int launch_step_t(const StepLaunch& L, hipStream_t st)
+{
+    if (L.n_bc_list > 0) {
+        const int nb = (L.n_bc_list + 255) / 256;
+        hipLaunchKernelGGL((k_bc_prepass<S>), dim3(nb), dim3(256), 0, st, L);
+    }
+    const int bpg = L.bricks_per_group;
+    int groups = (L.n_list + bpg - 1) / bpg;
+    groups = ((groups + 7) / 8) * 8;
+    if (groups == 0) groups = 8;
+    hipLaunchKernelGGL((k_step<S, MODEL>), dim3(groups), dim3(64 * bpg), 0, st, L);
+    return check_launch();
+}
+
+template <int S>
+static int launch_step_s(int model, const StepLaunch& L, hipStream_t st)
+{
+    switch (model) {
+    case M_CUM: return launch_step_t<S, M_CUM>(L, st);
+    case M_BGK: return launch_step_t<S, M_BGK>(L, st);
+    case M_MRT: return launch_step_t<S, M_MRT>(L, st);
+    case M_KBC: return launch_step_t<S, M_KBC>(L, st);
+    default: g_kerr = "unknown collision"; return -1;
+    }
+}
+
+int launch_step(int storage, int model, int /*cy_mode*/, const StepLaunch& L, void* stream)
+{
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    switch (storage) {
+    case S_F64: return launch_step_s<S_F64>(model, L, st);
+    case S_F32: return launch_step_s<S_F32>(model, L, st);
+    case S_F16: return launch_step_s<S_F16>(model, L, st);
+    case S_BF16: return launch_step_s<S_BF16>(model, L, st);
+    default: g_kerr = "unknown storage"; return -1;
+    }
+}
+
+int launch_fill_populations(int storage, void* f, int64_t n_bricks, void* stream)
+{
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t n = n_bricks * kQ * kLanes;
+    const int nb = int((n + 255) / 256);
+    switch (storage) {
+    case S_F64: hipLaunchKernelGGL(k_fill_rest<S_F64>, dim3(nb), dim3(256), 0, st, static_cast<double*>(f), n); break;
+    case S_F32: hipLaunchKernelGGL(k_fill_rest<S_F32>, dim3(nb), dim3(256), 0, st, static_cast<float*>(f), n); break;
+    case S_F16: hipLaunchKernelGGL(k_fill_rest<S_F16>, dim3(nb), dim3(256), 0, st, static_cast<_Float16*>(f), n); break;
+    case S_BF16: hipLaunchKernelGGL(k_fill_rest<S_BF16>, dim3(nb), dim3(256), 0, st, static_cast<uint16_t*>(f), n); break;
+    default: return -1;
+    }
+    return check_launch();
+}
+
+int launch_strain(int storage, const CaseDev* dev_case, int n_active, void* stream)
+{
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (n_active <= 0) return 0;
+    if (storage == S_F64) hipLaunchKernelGGL(k_strain<double>, dim3(n_active), dim3(64), 0, st, dev_case);
+    else hipLaunchKernelGGL(k_strain<float>, dim3(n_active), dim3(64), 0, st, dev_case);
+    return check_launch();
+}
+
+int launch_flux(int storage, const CaseDev* dev_case, int n_bc, double* dev_out, void* stream)
+{
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (n_bc <= 0) return 0;
+    const int nb = (n_bc + 255) / 256;
+    if (storage == S_F64) hipLaunchKernelGGL(k_flux<double>, dim3(nb), dim3(256), 0, st, dev_case, dev_out);
+    else hipLaunchKernelGGL(k_flux<float>, dim3(nb), dim3(256), 0, st, dev_case, dev_out);
+    return check_launch();
+}
+
+int launch_zero(void* ptr, int64_t bytes, void* stream)
+{
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int64_t n = bytes / 4;
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_zero, dim3(int((n + 255) / 256)), dim3(256), 0, st, static_cast<uint32_t*>(ptr), n);
+    return check_launch();
+}
+
+int launch_set_u64(void* dev_ptr, uint64_t value, void* stream)
+{
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream),
+                       static_cast<uint64_t*>(dev_ptr), value);
+    return check_launch();
+}
+
+int launch_fd_force_prepass(int, const CaseDev*, int, void*)
+{
+    g_kerr = "fd-force coupling is not available in this build";
+    return -1;
+}
+
+}  // namespace lbm355

@@ -1,0 +1,161 @@
+"""Python face of liblbm355 (ctypes over include/lbm355.h).
+
+``Case`` holds one simulation on the GPU: the sim_tcpc label setup, the sparse
+brick index and the resident populations/statistics.  It mirrors what the
+reference's StateTcpc/LBM pair exposes to the sim_tcpc driver
+(solver/sim_tcpc/sim_tcpc.cu:438-838): per-step inflow hook, stepping,
+macroscopic fields, statistics reset and the 3D VTK output.
+
+There is no CPU fallback: every call goes to the HIP library and raises if it
+is not built or the device call fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Sequence
+
+import numpy as np
+
+from . import _abi
+from ._abi import FIELD, CaseDesc, CaseInfo, OutputDesc, check, load, make_desc
+
+
+class Case:
+    """One resident case.  ``geometry_flat`` is the x-fastest wall-mask label
+    array of the geometry VTK; keyword arguments are the CaseDesc fields."""
+
+    def __init__(self, geometry_flat: np.ndarray, dims: Sequence[int], **kw):
+        self.lib = load()
+        self.dims = tuple(int(v) for v in dims)
+        self.desc, self._geom = make_desc(geometry_flat, dims, **kw)
+        h = C.c_void_p()
+        rc = self.lib.lbm355_case_create(C.byref(self.desc), C.byref(h))
+        self.terminate = rc == _abi.ETERMINATE
+        if rc not in (_abi.OK, _abi.ETERMINATE):
+            check(rc, "lbm355_case_create")
+        self.h = h
+        self._geom = None  # the library copied what it needs
+
+    # -- lifetime
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.lbm355_case_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- stepping
+    def set_inflow(self, ivc: float, svc: float) -> None:
+        check(self.lib.lbm355_case_set_inflow(self.h, float(ivc), float(svc)), "set_inflow")
+
+    def step(self, n: int = 1) -> None:
+        check(self.lib.lbm355_case_step(self.h, int(n)), "step")
+
+    def sync(self) -> None:
+        check(self.lib.lbm355_case_sync(self.h), "sync")
+
+    def time_kernel(self, n: int) -> float:
+        ms = C.c_double()
+        check(self.lib.lbm355_case_time_kernel(self.h, int(n), C.byref(ms)), "time_kernel")
+        return ms.value
+
+    # -- outputs
+    def info(self) -> dict:
+        i = CaseInfo()
+        check(self.lib.lbm355_case_info_get(self.h, C.byref(i)), "info")
+        return i.as_dict()
+
+    def field(self, name: str) -> np.ndarray:
+        X, Y, Z = self.dims
+        out = np.empty(X * Y * Z, dtype=np.float64)
+        check(self.lib.lbm355_case_get_field(self.h, FIELD[name], out.ctypes.data_as(C.POINTER(C.c_double)),
+                                             out.nbytes), f"get_field({name})")
+        return out.reshape(Z, Y, X).transpose(2, 1, 0)
+
+    def populations(self) -> np.ndarray:
+        """[27, X, Y, Z] post-collision populations of the last step (A/B sense)."""
+        X, Y, Z = self.dims
+        out = np.empty(27 * X * Y * Z, dtype=np.float64)
+        check(self.lib.lbm355_case_get_populations(self.h, out.ctypes.data_as(C.POINTER(C.c_double)), out.nbytes),
+              "get_populations")
+        return out.reshape(27, Z, Y, X).transpose(0, 3, 2, 1)
+
+    def compute_strain(self) -> None:
+        check(self.lib.lbm355_case_compute_strain(self.h), "compute_strain")
+
+    def flux(self, mean: bool = False) -> np.ndarray:
+        out = (C.c_double * 4)()
+        check(self.lib.lbm355_case_flux(self.h, 1 if mean else 0, out), "flux")
+        return np.array(out[:], dtype=np.float64)
+
+    def reset_stats(self) -> None:
+        check(self.lib.lbm355_case_reset_stats(self.h), "reset_stats")
+
+    def write_vtk(self, path: str | Path, *, phys_density: float, phys_nu_inf: float, time: float, cycle: int,
+                  stat_steps: int) -> None:
+        o = OutputDesc()
+        o.phys_density = phys_density
+        o.phys_nu_inf = phys_nu_inf
+        o.time = time
+        o.cycle = int(cycle)
+        o.stat_steps = int(stat_steps)
+        check(self.lib.lbm355_case_write_vtk(self.h, str(path).encode(), C.byref(o)), "write_vtk")
+
+
+class Batch:
+    """Cases of one collision and storage type stepped by one launch per step."""
+
+    def __init__(self, cases: Sequence[Case], graph: bool = True):
+        self.lib = load()
+        self.cases = list(cases)
+        arr = (C.c_void_p * len(self.cases))(*[c.h for c in self.cases])
+        h = C.c_void_p()
+        check(self.lib.lbm355_batch_create(arr, len(self.cases), 1 if graph else 0, C.byref(h)), "batch_create")
+        self.h = h
+
+    def step(self, n: int) -> None:
+        check(self.lib.lbm355_batch_step(self.h, int(n)), "batch_step")
+
+    def set_shape(self, bricks_per_group: int) -> None:
+        check(self.lib.lbm355_batch_set_shape(self.h, int(bricks_per_group)), "batch_set_shape")
+
+    def autotune(self) -> int:
+        best = C.c_int()
+        check(self.lib.lbm355_batch_autotune(self.h, C.byref(best)), "batch_autotune")
+        return best.value
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.lbm355_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    n = C.c_int()
+    check(load().lbm355_device_count(C.byref(n)), "device_count")
+    return n.value
+
+
+def footprint(geometry_flat: np.ndarray, dims: Sequence[int], **kw) -> int:
+    d, keep = make_desc(geometry_flat, dims, **kw)
+    b = C.c_int64()
+    check(load().lbm355_case_footprint(C.byref(d), C.byref(b)), "footprint")
+    del keep
+    return b.value

@@ -1,0 +1,130 @@
+"""ctypes wrapper of the CPU oracle (oracle/lbm_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker, never as the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from nonnewlbm_amd._abi import FIELD, CaseDesc, CaseInfo, make_desc
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        build()
+    lib = C.CDLL(str(LIB))
+    D = C.POINTER(C.c_double)
+    lib.oracle_create.restype = C.c_void_p
+    lib.oracle_create.argtypes = [C.POINTER(CaseDesc)]
+    lib.oracle_destroy.argtypes = [C.c_void_p]
+    lib.oracle_step.argtypes = [C.c_void_p, C.c_int64]
+    lib.oracle_set_inflow.argtypes = [C.c_void_p, C.c_double, C.c_double]
+    lib.oracle_get_field.argtypes = [C.c_void_p, C.c_int, D]
+    lib.oracle_get_populations.argtypes = [C.c_void_p, D]
+    lib.oracle_compute_strain.argtypes = [C.c_void_p]
+    lib.oracle_info.argtypes = [C.c_void_p, C.POINTER(CaseInfo)]
+    lib.oracle_collide.argtypes = [C.c_int, C.c_int, D, C.c_double, D, D, C.c_double, D]
+    lib.oracle_feq.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, D]
+    lib.oracle_density_velocity.argtypes = [D, D, D]
+    lib.oracle_neq_nu.restype = C.c_double
+    lib.oracle_neq_nu.argtypes = [D, C.c_double, D, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+    lib.oracle_last_error.restype = C.c_char_p
+    _lib = lib
+    return lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class OracleCase:
+    """Dense A/B CPU restatement of one sim_tcpc case."""
+
+    def __init__(self, geometry_flat: np.ndarray, dims, **kw):
+        self.lib = load()
+        self.dims = tuple(int(v) for v in dims)
+        self.desc, self._keep = make_desc(geometry_flat, dims, **kw)
+        self.h = self.lib.oracle_create(C.byref(self.desc))
+        if not self.h:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.lib.oracle_destroy(h)
+            self.h = None
+
+    def step(self, n: int = 1) -> None:
+        self.lib.oracle_step(self.h, int(n))
+
+    def set_inflow(self, a: float, b: float) -> None:
+        self.lib.oracle_set_inflow(self.h, a, b)
+
+    def field(self, name: str) -> np.ndarray:
+        X, Y, Z = self.dims
+        out = np.empty(X * Y * Z, dtype=np.float64)
+        rc = self.lib.oracle_get_field(self.h, FIELD[name], _p(out))
+        if rc != 0:
+            raise RuntimeError(f"oracle field {name}")
+        return out.reshape(Z, Y, X).transpose(2, 1, 0)
+
+    def populations(self) -> np.ndarray:
+        X, Y, Z = self.dims
+        out = np.empty(27 * X * Y * Z, dtype=np.float64)
+        self.lib.oracle_get_populations(self.h, _p(out))
+        return out.reshape(27, Z, Y, X).transpose(0, 3, 2, 1)
+
+    def compute_strain(self) -> None:
+        self.lib.oracle_compute_strain(self.h)
+
+    def info(self) -> dict:
+        i = CaseInfo()
+        self.lib.oracle_info(self.h, C.byref(i))
+        return i.as_dict()
+
+
+def collide(model: int, f: np.ndarray, rho: float, u, F=(0.0, 0.0, 0.0), nu: float = 0.1, f64: bool = True) -> np.ndarray:
+    lib = load()
+    fin = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.empty(27)
+    uu = np.asarray(u, dtype=np.float64)
+    FF = np.asarray(F, dtype=np.float64)
+    lib.oracle_collide(model, 1 if f64 else 0, _p(fin), rho, _p(uu), _p(FF), nu, _p(out))
+    return out
+
+
+def feq(rho: float, ux: float, uy: float, uz: float) -> np.ndarray:
+    out = np.empty(27)
+    load().oracle_feq(rho, ux, uy, uz, _p(out))
+    return out
+
+
+def density_velocity(f: np.ndarray, F=(0.0, 0.0, 0.0)) -> np.ndarray:
+    fin = np.ascontiguousarray(f, dtype=np.float64)
+    FF = np.asarray(F, dtype=np.float64)
+    out = np.empty(4)
+    load().oracle_density_velocity(_p(fin), _p(FF), _p(out))
+    return out
+
+
+def neq_nu(f: np.ndarray, rho: float, u, nu_ref: float, nu0: float, lam: float, a: float, n: float) -> float:
+    fin = np.ascontiguousarray(f, dtype=np.float64)
+    uu = np.asarray(u, dtype=np.float64)
+    return float(load().oracle_neq_nu(_p(fin), rho, _p(uu), nu_ref, nu0, lam, a, n))
