@@ -403,7 +403,7 @@ int setup_device(lbm355_case& c)
     // BC slots in (brick, lane) order
     std::vector<int32_t> bc_base(size_t(std::max(na, 1)), -1), bc_read;
     std::vector<uint32_t> bc_wmask;
-    std::vector<float> bc_factor;
+    std::vector<double> bc_factor;
     c.bc_cell.clear();
     c.bc_label_h.clear();
     // inflow profile: centre and radius per x face
@@ -453,16 +453,16 @@ int setup_device(lbm355_case& c)
                 if (tx >= 0 && ty >= 0 && tz >= 0 && tx < X && ty < Y && tz < Z) wm |= 1u << q;
             }
             bc_wmask.push_back(wm);
-            float fac = 1.0f;
+            double fac = 1.0;
             if (is_inflow_label(lb)) {
                 const int side = lb == L_IN_XM ? 0 : 1;
                 if (d.profile == LBM355_PROFILE_PARABOLIC) {
                     const double r2 = ((y - cen[side][0]) * (y - cen[side][0]) + (z - cen[side][1]) * (z - cen[side][1])) /
                                       (rad[side] * rad[side]);
-                    fac = float(std::max(0.0, 1.0 - r2));
+                    fac = std::max(0.0, 1.0 - r2);
                 } else {
                     const int64_t area = L.area[side];
-                    fac = area > 0 ? float(1.0 / double(area)) : 0.0f;
+                    fac = area > 0 ? double(area) : 1.0e300;  // plug: u = flux / area
                 }
             }
             bc_factor.push_back(fac);
@@ -510,7 +510,8 @@ int setup_device(lbm355_case& c)
     h.bc_val = c.bc_val.p;
     h.bc_read = static_cast<const int32_t*>(c.bc_read.p);
     h.bc_wmask = static_cast<const uint32_t*>(c.bc_wmask.p);
-    h.bc_factor = static_cast<const float*>(c.bc_factor.p);
+    h.bc_factor = static_cast<const double*>(c.bc_factor.p);
+    h.plug = d.profile == LBM355_PROFILE_PARABOLIC ? 0 : 1;
     h.bc_label = static_cast<const uint8_t*>(c.bc_label.p);
     h.origin = static_cast<const int16_t*>(c.origin.p);
     h.step = static_cast<int64_t*>(c.step.p);

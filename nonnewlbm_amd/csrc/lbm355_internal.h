@@ -54,7 +54,7 @@ struct CaseDev {
     void* bc_val;             // [nbc][4] rho, ux, uy, uz (compute type)
     const int32_t* bc_read;   // [nbc][2][27] element offsets of the partner reads
     const uint32_t* bc_wmask; // [nbc] 27-bit mask of the odd-step writes inside the grid
-    const float* bc_factor;   // [nbc] inflow profile factor (1/area for plug)
+    const double* bc_factor;  // [nbc] parabolic profile factor, or the inlet area (plug)
     const uint8_t* bc_label;  // [nbc]
     const int16_t* origin;    // [active brick][3] brick coordinates (cells / 4)
     int64_t* step;            // completed steps
@@ -70,7 +70,7 @@ struct CaseDev {
     int32_t stats_every;      // 0 = off
     int64_t macro_step;       // write rho,u at this step index (-1 = never)
     int32_t macro_every;      // also every k steps (0 = never)
-    int32_t pad;
+    int32_t plug;             // 1: u = inflow / area (sim_tcpc.cu:298,306)
 };
 
 struct BrickRef {

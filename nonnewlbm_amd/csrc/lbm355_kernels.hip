@@ -562,7 +562,8 @@ __global__ void __launch_bounds__(256) k_bc_prepass(StepLaunch L)
         R pulse = R(1);
         if (C.pulse_amp != 0.0 && C.pulse_period > 0.0)
             pulse = R(1.0 + C.pulse_amp * sin(6.283185307179586 * double(step) / C.pulse_period));
-        const R u = (lab == L_IN_XM ? R(C.inflow_xm) : -R(C.inflow_xp)) * R(C.bc_factor[s]) * pulse;
+        const R base = lab == L_IN_XM ? R(C.inflow_xm) : -R(C.inflow_xp);
+        const R u = (C.plug ? base / R(C.bc_factor[s]) : base * R(C.bc_factor[s])) * pulse;
         out[0] = rho; out[1] = u; out[2] = R(0); out[3] = R(0);
     } else {
         R mx = R(0), my = R(0), mz = R(0);
