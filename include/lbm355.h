@@ -237,9 +237,17 @@ int lbm355_batch_autotune(lbm355_batch* b, int* best_bricks_per_group);
 int lbm355_case_footprint(const lbm355_case_desc* desc, int64_t* bytes);
 
 /* --- raw kernel timing (measurement only) ---------------------------------- */
-/* average duration in ms of the fused stream-collide launch over n steps,
-   timed with HIP events on the case's stream */
-int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_ms);
+/* n more time steps, timed with HIP events on the case's stream: avg_step_ms =
+   whole step (BC pre-pass + fused kernel) from one event pair around the n
+   steps; avg_kernel_ms = the fused stream-collide launch alone, from an event
+   pair around each launch of the first min(n, 512) steps.  No counterpart in
+   the reference, which only logs wall-clock MLUPS (core.h:1440-1469). */
+int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_step_ms, double* avg_kernel_ms);
+
+/* device-to-device copy bandwidth in GB/s (read + write bytes / time) of a
+   plain vectorised copy kernel over `bytes` bytes, best of `iters` runs:
+   the on-box reference for the roofline fractions */
+int lbm355_copy_bandwidth(int64_t bytes, int iters, double* gbps);
 
 #ifdef __cplusplus
 }

@@ -167,7 +167,8 @@ def load() -> C.CDLL:
         "lbm355_batch_set_shape": (C.c_int, [P, C.c_int]),
         "lbm355_batch_autotune": (C.c_int, [P, C.POINTER(C.c_int)]),
         "lbm355_case_footprint": (C.c_int, [C.POINTER(CaseDesc), C.POINTER(C.c_int64)]),
-        "lbm355_case_time_kernel": (C.c_int, [P, C.c_int64, C.POINTER(C.c_double)]),
+        "lbm355_case_time_kernel": (C.c_int, [P, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "lbm355_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -184,7 +185,7 @@ EXPORTED = [
     "lbm355_case_step", "lbm355_case_sync", "lbm355_case_get_field", "lbm355_case_get_populations",
     "lbm355_case_compute_strain", "lbm355_case_flux", "lbm355_case_reset_stats", "lbm355_case_write_vtk",
     "lbm355_batch_create", "lbm355_batch_step", "lbm355_batch_destroy", "lbm355_batch_set_shape",
-    "lbm355_batch_autotune", "lbm355_case_footprint", "lbm355_case_time_kernel",
+    "lbm355_batch_autotune", "lbm355_case_footprint", "lbm355_case_time_kernel", "lbm355_copy_bandwidth",
 ]
 
 

@@ -200,7 +200,7 @@ struct lbm355_case {
 
     hipStream_t stream = nullptr;
     DevBuf f, stats, macro, strain, force, nbr, label, bc_base, bc_val, bc_read, bc_wmask, bc_factor, bc_label,
-        origin, step, devcase, list, bc_list, done, flux;
+        origin, base, devcase, list, bc_list, flux;
     CaseDev host{};
     int bricks_per_group = 4;
 
@@ -337,8 +337,8 @@ int build_index(lbm355_case& c)
     std::sort(shell.begin(), shell.end());
     c.n_active = int32_t(act.size());
     c.n_stored = int32_t(act.size() + shell.size());
-    if (int64_t(c.n_stored + 1) * kQ * kLanes >= (int64_t(1) << 31))
-        return fail(LBM355_EINVAL, "case too large for 32-bit population offsets");
+    if (int64_t(c.n_stored + 1) * kQ * kLanes * int64_t(storage_bytes(d.storage)) >= (int64_t(1) << 32))
+        return fail(LBM355_EINVAL, "case too large: populations must stay below 4 GiB per case");
     c.brick_id.assign(NB, -1);
     c.brick_xyz.resize(size_t(c.n_stored) * 3);
     int32_t n = 0;
@@ -487,8 +487,7 @@ int setup_device(lbm355_case& c)
     if ((rc = alloc_zero(c, c.macro, size_t(std::max(na, 1)) * kMacroFields * kLanes * cb))) return rc;
     if ((rc = alloc_zero(c, c.strain, size_t(std::max(na, 1)) * 6 * kLanes * cb))) return rc;
     if (d.stats_every > 0 && (rc = alloc_zero(c, c.stats, size_t(std::max(na, 1)) * kStatFields * kLanes * 8))) return rc;
-    if ((rc = alloc_zero(c, c.step, 8))) return rc;
-    if ((rc = alloc_zero(c, c.done, 16))) return rc;
+    if ((rc = alloc_zero(c, c.base, 8))) return rc;
     if ((rc = alloc_zero(c, c.flux, 4 * 8))) return rc;
     std::vector<BrickRef> list(size_t(std::max(na, 1)));
     for (int b = 0; b < na; ++b) list[b] = BrickRef{0u, uint32_t(b)};
@@ -514,7 +513,9 @@ int setup_device(lbm355_case& c)
     h.plug = d.profile == LBM355_PROFILE_PARABOLIC ? 0 : 1;
     h.bc_label = static_cast<const uint8_t*>(c.bc_label.p);
     h.origin = static_cast<const int16_t*>(c.origin.p);
-    h.step = static_cast<int64_t*>(c.step.p);
+    h.start = 0;
+    h.f_bytes = int64_t(c.f.bytes);
+    h.stats_bytes = int64_t(c.stats.bytes);
     h.n_active = na;
     h.n_stored = ns;
     h.n_bc = c.n_bc;
@@ -563,7 +564,8 @@ StepLaunch single_launch(lbm355_case& c)
     L.n_list = c.n_active;
     L.cases = static_cast<CaseDev*>(c.devcase.p);
     L.n_cases = 1;
-    L.done = static_cast<uint32_t*>(c.done.p);
+    L.base = static_cast<const int64_t*>(c.base.p);
+    L.rel = c.steps;
     L.bricks_per_group = c.bricks_per_group;
     L.bc_list = static_cast<const uint2*>(c.bc_list.p);
     L.n_bc_list = c.n_bc;
@@ -697,7 +699,7 @@ int lbm355_case_info_get(const lbm355_case* c, lbm355_case_info* o)
     int64_t bytes = 0;
     for (const DevBuf* b : {&c->f, &c->stats, &c->macro, &c->strain, &c->force, &c->nbr, &c->label, &c->bc_base,
                             &c->bc_val, &c->bc_read, &c->bc_wmask, &c->bc_factor, &c->bc_label, &c->origin,
-                            &c->step, &c->devcase, &c->list, &c->bc_list, &c->done, &c->flux})
+                            &c->base, &c->devcase, &c->list, &c->bc_list, &c->flux})
         bytes += int64_t(b->bytes);
     o->device_bytes = bytes;
     o->steps = c->steps;
@@ -723,8 +725,11 @@ int lbm355_case_step(lbm355_case* c, int64_t n)
     int rc;
     // rho,u of the last step of this call are kept for get_field / VTK
     if ((rc = set_case_u64(*c, &c->host.macro_step, uint64_t(c->steps + n - 1)))) return rc;
-    const StepLaunch L = single_launch(*c);
-    for (int64_t s = 0; s < n; ++s) LAUNCH_TRY(launch_step(c->storage, c->model, c->host.cy_mode, L, c->stream));
+    StepLaunch L = single_launch(*c);
+    for (int64_t s = 0; s < n; ++s) {
+        L.rel = c->steps + s;
+        LAUNCH_TRY(launch_step(c->storage, c->model, 0, L, c->stream));
+    }
     if (c->d.stats_every > 0) {
         const int64_t k = c->d.stats_every;
         c->stats_steps += (c->steps + n) / k - c->steps / k;
@@ -742,27 +747,79 @@ int lbm355_case_sync(lbm355_case* c)
     return LBM355_OK;
 }
 
-int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_ms)
+int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_step_ms, double* avg_kernel_ms)
 {
-    if (!c || !avg_ms || n <= 0) return fail(LBM355_EINVAL, "bad argument");
-    hipEvent_t a, b;
-    HIP_TRY(hipEventCreate(&a));
-    HIP_TRY(hipEventCreate(&b));
-    const StepLaunch L = single_launch(*c);
-    HIP_TRY(hipEventRecord(a, c->stream));
-    for (int64_t s = 0; s < n; ++s) LAUNCH_TRY(launch_step(c->storage, c->model, c->host.cy_mode, L, c->stream));
-    HIP_TRY(hipEventRecord(b, c->stream));
-    HIP_TRY(hipEventSynchronize(b));
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    if (!c || n <= 0) return fail(LBM355_EINVAL, "bad argument");
+    int rc;
+    if ((rc = set_case_u64(*c, &c->host.macro_step, uint64_t(c->steps + n - 1)))) return rc;
+    StepLaunch L = single_launch(*c);
+    const int64_t nk = std::min<int64_t>(n, 512);
+    std::vector<hipEvent_t> ev(size_t(2 * nk + 2));
+    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    // pass 1: per-launch events around the fused kernel
+    for (int64_t s = 0; s < nk; ++s) {
+        L.rel = c->steps + s;
+        LAUNCH_TRY(launch_step(c->storage, c->model, 1, L, c->stream));
+        HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
+        LAUNCH_TRY(launch_step(c->storage, c->model, 2, L, c->stream));
+        HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
+    }
+    // pass 2: the remaining steps back to back between one event pair
+    HIP_TRY(hipEventRecord(ev[2 * nk], c->stream));
+    for (int64_t s = nk; s < n; ++s) {
+        L.rel = c->steps + s;
+        LAUNCH_TRY(launch_step(c->storage, c->model, 0, L, c->stream));
+    }
+    HIP_TRY(hipEventRecord(ev[2 * nk + 1], c->stream));
+    HIP_TRY(hipEventSynchronize(ev[2 * nk + 1]));
+    double ksum = 0.0;
+    for (int64_t s = 0; s < nk; ++s) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]));
+        ksum += ms;
+    }
+    float rest = 0;
+    HIP_TRY(hipEventElapsedTime(&rest, ev[2 * nk], ev[2 * nk + 1]));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (avg_kernel_ms) *avg_kernel_ms = ksum / double(nk);
+    if (avg_step_ms) *avg_step_ms = n > nk ? double(rest) / double(n - nk) : ksum / double(nk);
     if (c->d.stats_every > 0) {
         const int64_t k = c->d.stats_every;
         c->stats_steps += (c->steps + n) / k - c->steps / k;
     }
     c->steps += n;
-    *avg_ms = double(ms) / double(n);
+    c->macro_valid_step = c->steps - 1;
+    c->strain_valid = false;
+    return LBM355_OK;
+}
+
+int lbm355_copy_bandwidth(int64_t bytes, int iters, double* gbps)
+{
+    if (bytes < 1024 || iters < 1 || !gbps) return fail(LBM355_EINVAL, "bad argument");
+    bytes &= ~int64_t(15);
+    DevBuf a, b;
+    if (a.alloc(size_t(bytes)) || b.alloc(size_t(bytes))) return fail(LBM355_ENOMEM, "copy buffers");
+    hipStream_t st;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    LAUNCH_TRY(launch_zero(a.p, bytes, st));
+    LAUNCH_TRY(launch_copy(b.p, a.p, bytes, st));  // warm-up
+    double best = 0.0;
+    for (int i = 0; i < iters; ++i) {
+        HIP_TRY(hipEventRecord(e0, st));
+        LAUNCH_TRY(launch_copy(b.p, a.p, bytes, st));
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms > 0) best = std::max(best, 2.0 * double(bytes) / (double(ms) * 1e6));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    *gbps = best;
     return LBM355_OK;
 }
 
@@ -1037,8 +1094,11 @@ struct lbm355_batch {
     int use_graph = 0;
     int bricks_per_group = 4;
     hipStream_t stream = nullptr;
-    DevBuf devcases, list, bc_list, done;
+    DevBuf devcases, list, bc_list, base;
     int n_list = 0, n_bc = 0;
+    int64_t launches = 0;     // steps done by the batch
+    int64_t base_host = 0;    // mirror of the device launch base
+    std::vector<int64_t> start;  // per case: launch index of its step 0
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
@@ -1058,7 +1118,8 @@ StepLaunch batch_launch(lbm355_batch& b)
     L.n_list = b.n_list;
     L.cases = static_cast<CaseDev*>(b.devcases.p);
     L.n_cases = int32_t(b.cases.size());
-    L.done = static_cast<uint32_t*>(b.done.p);
+    L.base = static_cast<const int64_t*>(b.base.p);
+    L.rel = b.launches - b.base_host;
     L.bricks_per_group = b.bricks_per_group;
     L.bc_list = static_cast<const uint2*>(b.bc_list.p);
     L.n_bc_list = b.n_bc;
@@ -1094,8 +1155,13 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
         HIP_TRY(hipStreamSynchronize(c->stream));
         b->cases.push_back(c);
         cd.push_back(c->host);
+        cd.back().start = -c->steps;  // batch launch 0 is the case's step c->steps
+        b->start.push_back(-c->steps);
         for (int k = 0; k < c->n_active; ++k) list.push_back(BrickRef{uint32_t(i), uint32_t(k)});
+        // each wavefront of the BC pre-pass serves one case (its buffer
+        // descriptors are wave-uniform): pad every case's run to 64 entries
         for (int s = 0; s < c->n_bc; ++s) bl.push_back(make_uint2(uint32_t(i), uint32_t(s)));
+        while (bl.size() % kLanes) bl.push_back(make_uint2(uint32_t(i), 0xFFFFFFFFu));
     }
     // interleave the cases' Morton-ordered brick runs by chunks so that the
     // XCD-contiguous split of the list gives every XCD part of every case
@@ -1106,8 +1172,8 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
     if ((rc = upload(b->devcases, cd))) return rc;
     if ((rc = upload(b->list, list))) return rc;
     if ((rc = upload(b->bc_list, bl))) return rc;
-    if (b->done.alloc(16) != 0) return fail(LBM355_ENOMEM, "done counter");
-    HIP_TRY(hipMemset(b->done.p, 0, 16));
+    if (b->base.alloc(16) != 0) return fail(LBM355_ENOMEM, "launch counter");
+    LAUNCH_TRY(launch_zero(b->base.p, 16, b->stream));
     // macro output only on explicit request inside a batch
     for (int i = 0; i < n; ++i) {
         CaseDev* dc = static_cast<CaseDev*>(b->devcases.p) + i;
@@ -1130,41 +1196,67 @@ int lbm355_batch_step(lbm355_batch* b, int64_t n)
 {
     if (!b) return fail(LBM355_EINVAL, "null batch");
     if (n <= 0) return LBM355_OK;
-    const StepLaunch L = batch_launch(*b);
+    for (size_t i = 0; i < b->cases.size(); ++i) {
+        CaseDev* dc = static_cast<CaseDev*>(b->devcases.p) + i;
+        // a case stepped on its own since the last batch call: re-anchor it
+        const int64_t want = b->launches - b->cases[i]->steps;
+        if (want != b->start[i]) {
+            LAUNCH_TRY(launch_set_u64(&dc->start, uint64_t(want), b->stream));
+            b->start[i] = want;
+        }
+        // rho,u of every case at the last step of this call
+        LAUNCH_TRY(launch_set_u64(&dc->macro_step, uint64_t(b->cases[i]->steps + n - 1), b->stream));
+    }
     int64_t done = 0;
     if (b->use_graph) {
         const int chunk = 16;
         if (!b->exec) {
+            StepLaunch L = batch_launch(*b);
             HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            for (int s = 0; s < chunk; ++s) {
-                int r = launch_step(b->storage, b->model, 0, L, b->stream);
-                if (r) {
-                    hipGraph_t g;
-                    (void)hipStreamEndCapture(b->stream, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    return fail(LBM355_EDEVICE, kernel_error());
-                }
+            int r = 0;
+            for (int s = 0; s < chunk && r == 0; ++s) {
+                L.rel = s;
+                r = launch_step(b->storage, b->model, 0, L, b->stream);
             }
-            HIP_TRY(hipStreamEndCapture(b->stream, &b->graph));
+            if (r == 0) r = launch_advance(static_cast<int64_t*>(b->base.p), chunk, b->stream);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(b->stream, &g);
+            if (r != 0 || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(LBM355_EDEVICE, r ? kernel_error() : hipGetErrorString(e));
+            }
+            b->graph = g;
             HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
             b->graph_steps = chunk;
+        }
+        if (n >= b->graph_steps && b->base_host != b->launches) {
+            // host-driven steps since the last replay: move the device base up
+            LAUNCH_TRY(launch_advance(static_cast<int64_t*>(b->base.p), b->launches - b->base_host, b->stream));
+            b->base_host = b->launches;
         }
         while (n - done >= b->graph_steps) {
             HIP_TRY(hipGraphLaunch(b->exec, b->stream));
             done += b->graph_steps;
+            b->launches += b->graph_steps;
+            b->base_host += b->graph_steps;
         }
     }
-    for (; done < n; ++done) LAUNCH_TRY(launch_step(b->storage, b->model, 0, L, b->stream));
+    for (; done < n; ++done) {
+        const StepLaunch L = batch_launch(*b);
+        LAUNCH_TRY(launch_step(b->storage, b->model, 0, L, b->stream));
+        b->launches += 1;
+    }
     for (lbm355_case* c : b->cases) {
         if (c->d.stats_every > 0) {
             const int64_t k = c->d.stats_every;
             c->stats_steps += (c->steps + n) / k - c->steps / k;
         }
         c->steps += n;
-        c->macro_valid_step = -2;  // the batch does not write rho,u
+        c->macro_valid_step = -2;  // rho,u live in the batch's macro request; see batch_step
         c->strain_valid = false;
     }
     HIP_TRY(hipStreamSynchronize(b->stream));
+    for (lbm355_case* c : b->cases) c->macro_valid_step = c->steps - 1;
     return LBM355_OK;
 }
 

@@ -57,7 +57,9 @@ struct CaseDev {
     const double* bc_factor;  // [nbc] parabolic profile factor, or the inlet area (plug)
     const uint8_t* bc_label;  // [nbc]
     const int16_t* origin;    // [active brick][3] brick coordinates (cells / 4)
-    int64_t* step;            // completed steps
+    int64_t start;            // launch index of the case's step 0: step = launch - start
+    int64_t f_bytes;          // size of the population buffer (< 4 GiB)
+    int64_t stats_bytes;      // size of the statistics buffer (< 4 GiB)
     int32_t n_active;         // active bricks
     int32_t n_stored;         // stored bricks (dummy brick index = n_stored)
     int32_t n_bc;
@@ -84,7 +86,8 @@ struct StepLaunch {
     int32_t n_list;
     CaseDev* cases;           // device array
     int32_t n_cases;
-    uint32_t* done;           // device counter for the last-block step increment
+    const int64_t* base;      // device launch counter; this launch is *base + rel
+    int64_t rel;
     int32_t bricks_per_group; // 1, 2, 4, 8 wavefronts per workgroup
     // BC pre-pass list
     const uint2* bc_list;     // (case_id, bc slot)
@@ -108,7 +111,10 @@ __host__ __device__ constexpr double lattice_weight(int q)
 }
 
 // launchers (lbm355_kernels.hip); all asynchronous on `stream`
-int launch_step(int storage, int model, int cy_mode, const StepLaunch& L, void* stream);
+// phase 0: BC pre-pass + fused kernel, 1: pre-pass only, 2: fused kernel only
+int launch_step(int storage, int model, int phase, const StepLaunch& L, void* stream);
+int launch_copy(void* dst, const void* src, int64_t bytes, void* stream);
+int launch_advance(int64_t* base, int64_t k, void* stream);
 int launch_fill_populations(int storage, void* f, int64_t n_bricks, void* stream);
 int launch_strain(int storage, const CaseDev* dev_case, int n_active, void* stream);
 int launch_fd_force_prepass(int storage, const CaseDev* dev_case, int n_active, void* stream);

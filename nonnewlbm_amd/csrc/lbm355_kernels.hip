@@ -13,6 +13,7 @@
 // values and 6 FP64 statistics read-modify-writes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -33,7 +34,62 @@ __device__ __forceinline__ R wgt(int q)
 }
 
 // ---------------------------------------------------------------------------
-// storage policies: T is what lives in HBM, R the arithmetic type
+// buffer resources: populations and statistics are addressed through an SGPR
+// resource descriptor and a 32-bit per-lane byte offset (buffer_load/store
+// "offen"), which keeps the per-lane address math to one 32-bit multiply-add
+// and turns any offset past the end of the buffer into a zero read / dropped
+// write instead of a fault.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+
+// a wave-uniform copy of a pointer (readfirstlane of both halves), so that
+// descriptors built from it stay in SGPRs even inside divergent code
+template <class T>
+__device__ __forceinline__ T* uniform(T* p)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return reinterpret_cast<T*>((uint64_t(hi) << 32) | lo);
+}
+
+// descriptor word 3 of a raw buffer on the gfx9 family: DATA_FORMAT = 32
+// bits (a zero format makes every access out of range)
+constexpr int kRsrcWord3 = 0x00020000;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, int64_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(uniform(p)), (short)0,
+                                             __builtin_amdgcn_readfirstlane(int(uint32_t(bytes))), kRsrcWord3);
+}
+__device__ __forceinline__ float ld_f32(rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void st_f32(rsrc_t r, uint32_t off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
+}
+__device__ __forceinline__ double ld_f64(rsrc_t r, uint32_t off)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void st_f64(rsrc_t r, uint32_t off, double v)
+{
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+}
+__device__ __forceinline__ uint16_t ld_u16(rsrc_t r, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+}
+__device__ __forceinline__ void st_u16(rsrc_t r, uint32_t off, uint16_t v)
+{
+    __builtin_amdgcn_raw_buffer_store_b16(v, r, off, 0, 0);
+}
+
+// ---------------------------------------------------------------------------
+// storage policies: T is what lives in HBM, R the arithmetic type; ld/st take
+// the element index i (converted to a byte offset here) and the direction q
 template <int S>
 struct Pop;
 
@@ -41,16 +97,19 @@ template <>
 struct Pop<S_F64> {
     using T = double;
     using R = double;
-    __device__ static R ld(const T* p, int i, int) { return p[i]; }
-    __device__ static void st(T* p, int i, int, R v) { p[i] = v; }
+    __device__ static R ld(rsrc_t r, uint32_t i, int) { return ld_f64(r, i * 8u); }
+    __device__ static void st(rsrc_t r, uint32_t i, int, R v) { st_f64(r, i * 8u, v); }
+    // plain-pointer variants for the fill kernel
+    __device__ static void put(T* p, int64_t i, int, R v) { p[i] = v; }
 };
 
 template <>
 struct Pop<S_F32> {
     using T = float;
     using R = float;
-    __device__ static R ld(const T* p, int i, int) { return p[i]; }
-    __device__ static void st(T* p, int i, int, R v) { p[i] = v; }
+    __device__ static R ld(rsrc_t r, uint32_t i, int) { return ld_f32(r, i * 4u); }
+    __device__ static void st(rsrc_t r, uint32_t i, int, R v) { st_f32(r, i * 4u, v); }
+    __device__ static void put(T* p, int64_t i, int, R v) { p[i] = v; }
 };
 
 // f_q - w_q in IEEE half: the deviation from the rest state is what needs
@@ -59,25 +118,57 @@ template <>
 struct Pop<S_F16> {
     using T = _Float16;
     using R = float;
-    __device__ static R ld(const T* p, int i, int q) { return float(p[i]) + wgt<float>(q); }
-    __device__ static void st(T* p, int i, int q, R v) { p[i] = _Float16(v - wgt<float>(q)); }
+    __device__ static R ld(rsrc_t r, uint32_t i, int q)
+    {
+        return float(__builtin_bit_cast(_Float16, ld_u16(r, i * 2u))) + wgt<float>(q);
+    }
+    __device__ static void st(rsrc_t r, uint32_t i, int q, R v)
+    {
+        st_u16(r, i * 2u, __builtin_bit_cast(uint16_t, _Float16(v - wgt<float>(q))));
+    }
+    __device__ static void put(T* p, int64_t i, int q, R v) { p[i] = _Float16(v - wgt<float>(q)); }
 };
 
 // f_q - w_q in bfloat16 (round to nearest even on store)
+__device__ __forceinline__ uint16_t to_bf16(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
 template <>
 struct Pop<S_BF16> {
     using T = uint16_t;
     using R = float;
-    __device__ static R ld(const T* p, int i, int q)
+    __device__ static R ld(rsrc_t r, uint32_t i, int q)
     {
-        return __uint_as_float(uint32_t(p[i]) << 16) + wgt<float>(q);
+        return __uint_as_float(uint32_t(ld_u16(r, i * 2u)) << 16) + wgt<float>(q);
     }
-    __device__ static void st(T* p, int i, int q, R v)
-    {
-        uint32_t u = __float_as_uint(v - wgt<float>(q));
-        u += 0x7FFFu + ((u >> 16) & 1u);
-        p[i] = uint16_t(u >> 16);
-    }
+    __device__ static void st(rsrc_t r, uint32_t i, int q, R v) { st_u16(r, i * 2u, to_bf16(v - wgt<float>(q))); }
+    __device__ static void put(T* p, int64_t i, int q, R v) { p[i] = to_bf16(v - wgt<float>(q)); }
+};
+
+// ---------------------------------------------------------------------------
+// arithmetic policy: FP64 (the parity path) uses correctly rounded division,
+// sqrt and pow; FP32 uses the hardware reciprocal, square root, log2 and exp2
+// (about 1 ulp), which keeps the collision and the Carreau-Yasuda solve a few
+// instructions each.
+template <class R>
+struct Fm;
+template <>
+struct Fm<double> {
+    __device__ static double rcp(double x) { return 1.0 / x; }
+    __device__ static double div(double a, double b) { return a / b; }
+    __device__ static double sqrt_(double x) { return sqrt(x); }
+    __device__ static double pow_(double b, double e) { return pow(b, e); }
+};
+template <>
+struct Fm<float> {
+    __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+    // b >= 0: exp2(e log2 b); b = 0 gives exp2(-inf) = 0 for e > 0
+    __device__ static float pow_(float b, float e) { return __builtin_amdgcn_exp2f(e * __builtin_amdgcn_logf(b)); }
 };
 
 // ---------------------------------------------------------------------------
@@ -124,7 +215,7 @@ __device__ __forceinline__ void collide_cumulant(R (&m)[27], R rho, R ux, R uy, 
 #pragma unroll
     for (int b = 0; b < 9; ++b) to_central(m[b], m[9 + b], m[18 + b], ux);
 
-    const R inv = R(1) / rho;
+    const R inv = Fm<R>::rcp(rho);
     const R om1 = R(1) - om;
     const R k000 = m[K(0, 0, 0)];
     const R c110 = om1 * m[K(1, 1, 0)];
@@ -199,7 +290,7 @@ __device__ __forceinline__ void collide_bgk(R (&f)[27], R rho, R ux, R uy, R uz,
 {
     R px[3], py[3], pz[3];
     phi3(ux, px); phi3(uy, py); phi3(uz, pz);
-    const R src = (R(1) - R(0.5) * om) * R(3) / rho;
+    const R src = (R(1) - R(0.5) * om) * R(3) * Fm<R>::rcp(rho);
     const R uF = ux * Fx + uy * Fy + uz * Fz;
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
@@ -226,7 +317,7 @@ __device__ __forceinline__ void collide_mrt(R (&f)[27], R rho, R ux, R uy, R uz,
     const R nxy = Pxy - rho * ux * uy, nxz = Pxz - rho * ux * uz, nyz = Pyz - rho * uy * uz;
     const R Q = R(2) * (nxx * nxx + nyy * nyy + nzz * nzz + R(2) * (nxy * nxy + nxz * nxz + nyz * nyz));
     const R tau = R(3) * nu + R(0.5);
-    const R om = R(2) / (sqrt(tau * tau + R(18) * R(0.0342) * sqrt(Q) / rho) + tau);
+    const R om = Fm<R>::div(R(2), Fm<R>::sqrt_(tau * tau + R(18) * R(0.0342) * Fm<R>::sqrt_(Q) * Fm<R>::rcp(rho)) + tau);
     Pxx -= om * nxx; Pyy -= om * nyy; Pzz -= om * nzz;
     Pxy -= om * nxy; Pxz -= om * nxz; Pyz -= om * nyz;
     const R tr = Pxx + Pyy + Pzz;
@@ -296,15 +387,15 @@ __device__ __forceinline__ void collide_kbc(R (&f)[27], R rho, R ux, R uy, R uz,
         const R feq = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
         const R ds = kbc_shear<R>(q, M2, M3);
         const R dh = f[q] - feq - ds;
-        const R ifeq = R(1) / feq;
+        const R ifeq = Fm<R>::rcp(feq);
         num += ds * dh * ifeq;
         den += dh * dh * ifeq;
     }
-    const R beta = R(1) / (R(6) * nu + R(1));
-    const R ib = R(1) / beta;
+    const R beta = Fm<R>::rcp(R(6) * nu + R(1));
+    const R ib = R(6) * nu + R(1);
     // 0/0 only when dh == 0 everywhere, where any finite gamma gives the same f
-    const R gamma = den != R(0) ? ib - (R(2) - ib) * num / den : R(2);
-    const R src = (R(1) - beta) * R(3) / rho;
+    const R gamma = den != R(0) ? ib - (R(2) - ib) * Fm<R>::div(num, den) : R(2);
+    const R src = (R(1) - beta) * R(3) * Fm<R>::rcp(rho);
     const R uF = ux * Fx + uy * Fy + uz * Fz;
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
@@ -318,9 +409,7 @@ __device__ __forceinline__ void collide_kbc(R (&f)[27], R rho, R ux, R uy, R uz,
 
 // Carreau-Yasuda: nu = nu_ref + (nu0 - nu_ref) (1 + (lambda gamma)^a)^((n-1)/a)
 // (solver/lbm3d/core.h:766, sim_tcpc.cu:697-707)
-__device__ __forceinline__ double cy_pow(double b, double e) { return pow(b, e); }
-__device__ __forceinline__ float cy_pow(float b, float e) { return powf(b, e); }
-
+//
 // local viscosity of the "neq" coupling: gamma from the deviatoric
 // non-equilibrium second moment P, gamma * tau(gamma) = 3 sqrt(2)|P|/(2 rho),
 // solved by 8 safeguarded Newton steps (the same iteration as the oracle)
@@ -336,30 +425,30 @@ __device__ __forceinline__ R neq_viscosity(const R (&f)[27], R rho, R ux, R uy, 
     }
     Pxx -= rho * ux * ux; Pyy -= rho * uy * uy; Pzz -= rho * uz * uz;
     Pxy -= rho * ux * uy; Pxz -= rho * ux * uz; Pyz -= rho * uy * uz;
-    const R tr = (Pxx + Pyy + Pzz) / R(3);
+    const R tr = (Pxx + Pyy + Pzz) * (R(1) / R(3));
     const R dxx = Pxx - tr, dyy = Pyy - tr, dzz = Pzz - tr;
     const R s2 = dxx * dxx + dyy * dyy + dzz * dzz + R(2) * (Pxy * Pxy + Pxz * Pxz + Pyz * Pyz);
-    const R G = R(3) * sqrt(R(2)) * sqrt(s2) / (R(2) * rho);
+    const R G = R(3) * Fm<R>::sqrt_(R(2)) * Fm<R>::sqrt_(s2) * Fm<R>::rcp(R(2) * rho);
     const R nr = R(C.nu_ref), n0 = R(C.cy_nu0), lam = R(C.cy_lambda), a = R(C.cy_a), n = R(C.cy_n);
     const R e = (n - R(1)) / a;
     const R tr_ = R(3) * nr + R(0.5), t0 = R(3) * n0 + R(0.5);
-    R lo = G / fmax(tr_, t0), hi = G / fmin(tr_, t0);
+    R lo = Fm<R>::div(G, fmax(tr_, t0)), hi = Fm<R>::div(G, fmin(tr_, t0));
     R g = hi;
 #pragma unroll 1
     for (int it = 0; it < 8; ++it) {
-        const R x = cy_pow(lam * g, a);
+        const R x = Fm<R>::pow_(lam * g, a);
         const R b = R(1) + x;
-        const R pw = cy_pow(b, e);
+        const R pw = Fm<R>::pow_(b, e);
         const R nu = nr + (n0 - nr) * pw;
         const R h = g * (R(3) * nu + R(0.5)) - G;
-        const R dgnu = nr + (n0 - nr) * pw * (R(1) + (n - R(1)) * x / b);
+        const R dgnu = nr + (n0 - nr) * pw * (R(1) + (n - R(1)) * Fm<R>::div(x, b));
         const R dh = R(3) * dgnu + R(0.5);
         if (h > R(0)) hi = g; else lo = g;
-        R gn = g - h / dh;
+        R gn = g - Fm<R>::div(h, dh);
         if (!(gn > lo && gn < hi)) gn = R(0.5) * (lo + hi);
         g = gn;
     }
-    return nr + (n0 - nr) * cy_pow(R(1) + cy_pow(lam * g, a), e);
+    return nr + (n0 - nr) * Fm<R>::pow_(R(1) + Fm<R>::pow_(lam * g, a), e);
 }
 
 // ---------------------------------------------------------------------------
@@ -386,23 +475,34 @@ template <int S, int MODEL>
 __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef br, int lane)
 {
     using P = Pop<S>;
-    using T = typename P::T;
     using R = typename P::R;
     const CaseDev& C = L.cases[br.case_id];
     const int b = int(br.brick);
-    T* __restrict__ F = static_cast<T*>(C.f);
-    const int64_t step = *C.step;
+    const rsrc_t F = make_rsrc(C.f, C.f_bytes);
+    const int64_t step = *L.base + L.rel - C.start;
     const int parity = int(step & 1);
-    const int lab = C.label[b * kLanes + lane];
+    // per-step output decisions, wave-uniform (32-bit step arithmetic)
+    const uint32_t next = uint32_t(step) + 1u;
+    const bool do_stats = C.stats && C.stats_every > 0 &&
+                          (C.stats_every == 1 || next % uint32_t(C.stats_every) == 0u);
+    const bool do_macro = C.macro && (step == C.macro_step ||
+                                      (C.macro_every > 0 && next % uint32_t(C.macro_every) == 0u));
+    const rsrc_t SR = make_rsrc(C.stats, C.stats_bytes);
+    R* const mo = uniform(static_cast<R*>(C.macro));
+    const int lab = ((const __attribute__((address_space(1))) uint8_t*)C.label)[b * kLanes + lane];
     const bool fluid = lab == L_FLUID;
     const bool bc = is_bc_label(lab);
     const uint64_t bc_lanes = __ballot(bc);
 
+    // the 27 neighbour bricks are wave-uniform: scalar loads into SGPRs
+    const __attribute__((address_space(4))) int32_t* nbr_c =
+        (const __attribute__((address_space(4))) int32_t*)C.nbr + b * 27;
     int t[27];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) t[q] = C.nbr[b * 27 + q];
+    for (int q = 0; q < 27; ++q) t[q] = nbr_c[q];
 
     const int i = lane & 3, j = (lane >> 2) & 3, k = lane >> 4;
+    const uint32_t own = uint32_t(b * 27) * kLanes + uint32_t(lane);
 
     R f[27];
     R rho = R(1), ux = R(0), uy = R(0), uz = R(0);
@@ -410,7 +510,7 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
     if (fluid) {
         if (parity == 0) {
 #pragma unroll
-            for (int q = 0; q < 27; ++q) f[q] = P::ld(F, (b * 27 + q) * kLanes + lane, q);
+            for (int q = 0; q < 27; ++q) f[q] = P::ld(F, own + uint32_t(q * kLanes), q);
         } else {
 #pragma unroll
             for (int q = 0; q < 27; ++q) {
@@ -418,7 +518,7 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
                 const int si = i - cx, sj = j - cy, sk = k - cz;
                 const int nb = pick_brick(t, -cx, -cy, -cz, (si & ~3) != 0, (sj & ~3) != 0, (sk & ~3) != 0);
                 const int l2 = (si & 3) | ((sj & 3) << 2) | ((sk & 3) << 4);
-                f[q] = P::ld(F, (nb * 27 + (26 - q)) * kLanes + l2, q);
+                f[q] = P::ld(F, uint32_t(nb * 27 + (26 - q)) * kLanes + uint32_t(l2), q);
             }
         }
         R Fx = R(0), Fy = R(0), Fz = R(0);
@@ -439,13 +539,13 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
             const R d = f[26 - q] - f[q];  // direction 26-q has c = -c_q
             mx -= R(qx(q)) * d; my -= R(qy(q)) * d; mz -= R(qz(q)) * d;
         }
-        const R ir = R(1) / rho;
+        const R ir = Fm<R>::rcp(rho);
         ux = (mx + R(0.5) * Fx) * ir;
         uy = (my + R(0.5) * Fy) * ir;
         uz = (mz + R(0.5) * Fz) * ir;
         R nu = R(C.nu_ref);
         if (C.cy_mode == 1) nu = neq_viscosity<R>(f, rho, ux, uy, uz, C);
-        const R om = R(1) / (R(3) * nu + R(0.5));
+        const R om = Fm<R>::rcp(R(3) * nu + R(0.5));
         if constexpr (MODEL == M_CUM) collide_cumulant<R>(f, rho, ux, uy, uz, om);
         else if constexpr (MODEL == M_BGK) collide_bgk<R>(f, rho, ux, uy, uz, Fx, Fy, Fz, om);
         else if constexpr (MODEL == M_MRT) collide_mrt<R>(f, rho, ux, uy, uz, nu);
@@ -461,7 +561,7 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
     if (fluid || bc) {
         if (parity == 0) {
 #pragma unroll
-            for (int q = 0; q < 27; ++q) P::st(F, (b * 27 + (26 - q)) * kLanes + lane, q, f[q]);
+            for (int q = 0; q < 27; ++q) P::st(F, own + uint32_t((26 - q) * kLanes), q, f[q]);
         } else {
 #pragma unroll
             for (int q = 0; q < 27; ++q) {
@@ -469,26 +569,27 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
                 const int di = i + cx, dj = j + cy, dk = k + cz;
                 const int nb = pick_brick(t, cx, cy, cz, (di & ~3) != 0, (dj & ~3) != 0, (dk & ~3) != 0);
                 const int l2 = (di & 3) | ((dj & 3) << 2) | ((dk & 3) << 4);
-                if ((wmask >> q) & 1u) P::st(F, (nb * 27 + q) * kLanes + l2, q, f[q]);
+                // BC cells skip the writes that would leave the grid; the
+                // buffer resource drops them anyway, this keeps them off the bus
+                if ((wmask >> q) & 1u) P::st(F, uint32_t(nb * 27 + q) * kLanes + uint32_t(l2), q, f[q]);
             }
         }
         // macroscopic output (MacroTcpc::outputMacro, sim_tcpc.cu:337-354)
-        if (C.macro && (step == C.macro_step || (C.macro_every > 0 && (step + 1) % C.macro_every == 0))) {
-            R* mo = static_cast<R*>(C.macro);
+        if (do_macro) {
             mo[(b * 4 + 0) * kLanes + lane] = rho;
             mo[(b * 4 + 1) * kLanes + lane] = ux;
             mo[(b * 4 + 2) * kLanes + lane] = uy;
             mo[(b * 4 + 3) * kLanes + lane] = uz;
         }
-        if (C.stats && C.stats_every > 0 && (step + 1) % C.stats_every == 0) {
-            double* st = C.stats + (size_t)b * kStatFields * kLanes + lane;
+        if (do_stats) {
+            const uint32_t o = (uint32_t(b * kStatFields) * kLanes + uint32_t(lane)) * 8u;
             const double rx = double(rho) * double(ux), ry = double(rho) * double(uy), rz = double(rho) * double(uz);
-            st[0 * kLanes] += rx;
-            st[1 * kLanes] += ry;
-            st[2 * kLanes] += rz;
-            st[3 * kLanes] += rx * rx;
-            st[4 * kLanes] += ry * ry;
-            st[5 * kLanes] += rz * rz;
+            const double add[6] = {rx, ry, rz, rx * rx, ry * ry, rz * rz};
+            double cur[6];
+#pragma unroll
+            for (int a = 0; a < 6; ++a) cur[a] = ld_f64(SR, o + uint32_t(a * kLanes * 8));
+#pragma unroll
+            for (int a = 0; a < 6; ++a) st_f64(SR, o + uint32_t(a * kLanes * 8), cur[a] + add[a]);
         }
     }
 }
@@ -502,22 +603,11 @@ __device__ __forceinline__ int xcd_block(int bid, int grid)
     return (bid & 7) * per + (bid >> 3);
 }
 
-// every case of the launch has finished this step once the last workgroup is
-// done: that workgroup advances the step counters (no waiting on other
-// workgroups, so no hang is possible)
-__device__ __forceinline__ void finish_step(const StepLaunch& L)
-{
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(L.done, 1u);
-        if (prev == gridDim.x - 1) {
-            for (int c = 0; c < L.n_cases; ++c) *L.cases[c].step += 1;
-            *L.done = 0u;
-            __threadfence();
-        }
-    }
-}
+// the step index of a launch is *base + rel: host-driven launches pass it in
+// rel (base stays put), a captured graph of K steps bakes rel = 0..K-1 and
+// ends with k_advance, so replays need no host update and no device-wide
+// atomics or fences inside the step kernels
+__global__ void k_advance(int64_t* base, int64_t k) { *base += k; }
 
 template <int S, int MODEL>
 __global__ void __launch_bounds__(512) k_step(StepLaunch L)
@@ -529,7 +619,6 @@ __global__ void __launch_bounds__(512) k_step(StepLaunch L)
         const BrickRef br = L.list[w];
         step_brick<S, MODEL>(L, br, int(threadIdx.x) & 63);
     }
-    finish_step(L);
 }
 
 // BC pre-pass: rho (inflow) or u (outflow) from the populations that enter
@@ -539,20 +628,20 @@ template <int S>
 __global__ void __launch_bounds__(256) k_bc_prepass(StepLaunch L)
 {
     using P = Pop<S>;
-    using T = typename P::T;
     using R = typename P::R;
     const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= L.n_bc_list) return;
     const uint2 e = L.bc_list[t];
+    if (e.y == 0xFFFFFFFFu) return;  // padding: each wavefront serves one case
     const CaseDev& C = L.cases[e.x];
     const int s = int(e.y);
-    const int64_t step = *C.step;
+    const int64_t step = *L.base + L.rel - C.start;
     const int parity = int(step & 1);
-    const T* F = static_cast<const T*>(C.f);
+    const rsrc_t F = make_rsrc(C.f, C.f_bytes);
     const int32_t* rd = C.bc_read + (s * 2 + parity) * 27;
     R g[27];
 #pragma unroll
-    for (int q = 0; q < 27; ++q) g[q] = P::ld(F, rd[q], q);
+    for (int q = 0; q < 27; ++q) g[q] = P::ld(F, uint32_t(rd[q]), q);
     R rho = g[13];
 #pragma unroll
     for (int q = 0; q < 13; ++q) rho += g[q] + g[26 - q];
@@ -587,7 +676,7 @@ __global__ void k_fill_rest(typename Pop<S>::T* f, int64_t n)
     if (i >= n) return;
     const int q = int((i / kLanes) % kQ);
     // rest state f = f_eq(1, 0) = w_q (resetLattice, sim_tcpc.cu:647-655)
-    P::st(f + (i / kLanes) * kLanes, int(i % kLanes), q, R(lattice_weight(q)));
+    P::put(f, i, q, R(lattice_weight(q)));
 }
 
 __global__ void k_zero(uint32_t* p, int64_t n)
@@ -717,12 +806,13 @@ static int check_launch()
 }
 
 template <int S, int MODEL>
-static int launch_step_t(const StepLaunch& L, hipStream_t st)
+static int launch_step_t(const StepLaunch& L, int phase, hipStream_t st)
 {
-    if (L.n_bc_list > 0) {
+    if (phase != 2 && L.n_bc_list > 0) {
         const int nb = (L.n_bc_list + 255) / 256;
         hipLaunchKernelGGL((k_bc_prepass<S>), dim3(nb), dim3(256), 0, st, L);
     }
+    if (phase == 1) return check_launch();
     const int bpg = L.bricks_per_group;
     int groups = (L.n_list + bpg - 1) / bpg;
     groups = ((groups + 7) / 8) * 8;
@@ -732,25 +822,25 @@ static int launch_step_t(const StepLaunch& L, hipStream_t st)
 }
 
 template <int S>
-static int launch_step_s(int model, const StepLaunch& L, hipStream_t st)
+static int launch_step_s(int model, const StepLaunch& L, int phase, hipStream_t st)
 {
     switch (model) {
-    case M_CUM: return launch_step_t<S, M_CUM>(L, st);
-    case M_BGK: return launch_step_t<S, M_BGK>(L, st);
-    case M_MRT: return launch_step_t<S, M_MRT>(L, st);
-    case M_KBC: return launch_step_t<S, M_KBC>(L, st);
+    case M_CUM: return launch_step_t<S, M_CUM>(L, phase, st);
+    case M_BGK: return launch_step_t<S, M_BGK>(L, phase, st);
+    case M_MRT: return launch_step_t<S, M_MRT>(L, phase, st);
+    case M_KBC: return launch_step_t<S, M_KBC>(L, phase, st);
     default: g_kerr = "unknown collision"; return -1;
     }
 }
 
-int launch_step(int storage, int model, int /*cy_mode*/, const StepLaunch& L, void* stream)
+int launch_step(int storage, int model, int phase, const StepLaunch& L, void* stream)
 {
     hipStream_t st = static_cast<hipStream_t>(stream);
     switch (storage) {
-    case S_F64: return launch_step_s<S_F64>(model, L, st);
-    case S_F32: return launch_step_s<S_F32>(model, L, st);
-    case S_F16: return launch_step_s<S_F16>(model, L, st);
-    case S_BF16: return launch_step_s<S_BF16>(model, L, st);
+    case S_F64: return launch_step_s<S_F64>(model, L, phase, st);
+    case S_F32: return launch_step_s<S_F32>(model, L, phase, st);
+    case S_F16: return launch_step_s<S_F16>(model, L, phase, st);
+    case S_BF16: return launch_step_s<S_BF16>(model, L, phase, st);
     default: g_kerr = "unknown storage"; return -1;
     }
 }
@@ -802,6 +892,29 @@ int launch_set_u64(void* dev_ptr, uint64_t value, void* stream)
 {
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream),
                        static_cast<uint64_t*>(dev_ptr), value);
+    return check_launch();
+}
+
+// plain vectorised copy, the on-box bandwidth reference
+__global__ void __launch_bounds__(256) k_copy(const float4* __restrict__ src, float4* __restrict__ dst, int64_t n)
+{
+    const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+int launch_copy(void* dst, const void* src, int64_t bytes, void* stream)
+{
+    const int64_t n = bytes / 16;
+    int groups = int(std::min<int64_t>((n + 255) / 256, 256 * 64));
+    if (groups < 1) groups = 1;
+    hipLaunchKernelGGL(k_copy, dim3(groups), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       static_cast<const float4*>(src), static_cast<float4*>(dst), n);
+    return check_launch();
+}
+
+int launch_advance(int64_t* base, int64_t k, void* stream)
+{
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), base, k);
     return check_launch();
 }
 

@@ -65,10 +65,11 @@ class Case:
     def sync(self) -> None:
         check(self.lib.lbm355_case_sync(self.h), "sync")
 
-    def time_kernel(self, n: int) -> float:
-        ms = C.c_double()
-        check(self.lib.lbm355_case_time_kernel(self.h, int(n), C.byref(ms)), "time_kernel")
-        return ms.value
+    def time_kernel(self, n: int) -> tuple[float, float]:
+        """Run n steps; returns (ms per whole step, ms per fused-kernel launch)."""
+        step_ms, kern_ms = C.c_double(), C.c_double()
+        check(self.lib.lbm355_case_time_kernel(self.h, int(n), C.byref(step_ms), C.byref(kern_ms)), "time_kernel")
+        return step_ms.value, kern_ms.value
 
     # -- outputs
     def info(self) -> dict:
@@ -151,6 +152,13 @@ def device_count() -> int:
     n = C.c_int()
     check(load().lbm355_device_count(C.byref(n)), "device_count")
     return n.value
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 10) -> float:
+    """GB/s of a plain device copy kernel (read + write bytes per second)."""
+    g = C.c_double()
+    check(load().lbm355_copy_bandwidth(int(nbytes), int(iters), C.byref(g)), "copy_bandwidth")
+    return g.value
 
 
 def footprint(geometry_flat: np.ndarray, dims: Sequence[int], **kw) -> int:
