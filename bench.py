@@ -110,7 +110,7 @@ def main() -> int:
     mlups = active * args.steps / wall / 1e6
     print(f"# timed {args.steps} steps in {wall:.3f} s -> {mlups:.1f} fluid MLUPS", file=sys.stderr, flush=True)
 
-    step_ms, kern_ms = case.time_kernel(512)
+    step_ms, kern_ms = case.time_kernel(1024)
     rho = case.field("rho")
     vx = case.field("vx")
     lab = case.field("label")
@@ -170,7 +170,7 @@ def main() -> int:
         c2 = Case(g2.flat(), g2.dims, **kw2)
         i2 = c2.info()
         c2.step(50)
-        s2, k2 = c2.time_kernel(300)
+        s2, k2 = c2.time_kernel(400)
         b2 = i2["active_cells"] * 2 * 27 * sbytes
         a2 = b2 / (k2 * 1e-3) / 1e9
         out["roofline_hbm_resident"] = {

@@ -186,6 +186,11 @@ struct lbm355_case {
     Labels lab;
     int storage = 1, model = 0;
     int nbx = 0, nby = 0, nbz = 0;
+    // brick origin shift: cell x sits in brick (x + ox) / 4.  Chosen so that
+    // the planes 0,1 and X-2,X-1 (and the same in y) share bricks: every BC
+    // cell then lives in the wavefront of the cells whose slots it reads,
+    // which lets the fused kernel read the BC partner populations itself.
+    int ox = 0, oy = 0;
     std::vector<int32_t> brick_id;      // brick grid -> storage index or -1
     std::vector<int32_t> brick_xyz;     // storage index -> (bx,by,bz)
     int32_t n_active = 0, n_stored = 0, n_bc = 0;
@@ -213,15 +218,22 @@ struct lbm355_case {
     int32_t stored_brick(int x, int y, int z) const
     {
         if (x < 0 || y < 0 || z < 0 || x >= d.nx || y >= d.ny || z >= d.nz) return -1;
-        return brick_id[(x >> 2) + int64_t(nbx) * ((y >> 2) + int64_t(nby) * (z >> 2))];
+        return brick_id[((x + ox) >> 2) + int64_t(nbx) * (((y + oy) >> 2) + int64_t(nby) * (z >> 2))];
     }
     // element offset of slot `slot` of cell (x,y,z); the dummy brick if absent
+    int lane_of(int x, int y, int z) const { return ((x + ox) & 3) | (((y + oy) & 3) << 2) | ((z & 3) << 4); }
+    void cell_of(int b, int l, int& x, int& y, int& z) const
+    {
+        x = brick_xyz[3 * b] * 4 + (l & 3) - ox;
+        y = brick_xyz[3 * b + 1] * 4 + ((l >> 2) & 3) - oy;
+        z = brick_xyz[3 * b + 2] * 4 + (l >> 4);
+    }
     int32_t slot_offset(int x, int y, int z, int slot) const
     {
         int32_t b = stored_brick(x, y, z);
         int lane = 0;
         if (b < 0) b = n_stored;
-        else lane = (x & 3) | ((y & 3) << 2) | ((z & 3) << 4);
+        else lane = lane_of(x, y, z);
         return (b * kQ + slot) * kLanes + lane;
     }
 };
@@ -293,12 +305,21 @@ int build_index(lbm355_case& c)
     const lbm355_case_desc& d = c.d;
     const int X = d.nx, Y = d.ny, Z = d.nz;
     const Labels& L = c.lab;
-    c.nbx = (X + 3) / 4; c.nby = (Y + 3) / 4; c.nbz = (Z + 3) / 4;
+    auto pick_shift = [](int n) {
+        for (int o = 0; o < 3; ++o)
+            if ((n - 1 + o) % 4 != 0) return o;
+        return 0;
+    };
+    c.ox = pick_shift(X);
+    c.oy = pick_shift(Y);
+    c.nbx = (X + c.ox + 3) / 4; c.nby = (Y + c.oy + 3) / 4; c.nbz = (Z + 3) / 4;
     const int64_t NB = int64_t(c.nbx) * c.nby * c.nbz;
     // bricks touched by the AA update: the 3x3x3 neighbourhood of every
     // non-wall cell, plus the partner reads of BC cells
     std::vector<uint8_t> active_b(NB, 0), stored_b(NB, 0);
-    auto bidx = [&](int x, int y, int z) { return (x >> 2) + int64_t(c.nbx) * ((y >> 2) + int64_t(c.nby) * (z >> 2)); };
+    auto bidx = [&](int x, int y, int z) {
+        return ((x + c.ox) >> 2) + int64_t(c.nbx) * (((y + c.oy) >> 2) + int64_t(c.nby) * (z >> 2));
+    };
     for (int z = 0; z < Z; ++z)
         for (int y = 0; y < Y; ++y)
             for (int x = 0; x < X; ++x) {
@@ -381,13 +402,13 @@ int setup_device(lbm355_case& c)
     std::vector<uint8_t> lab(size_t(ns + 1) * kLanes, L_WALL);
     for (int b = 0; b < ns; ++b)
         for (int l = 0; l < kLanes; ++l) {
-            const int x = c.brick_xyz[3 * b] * 4 + (l & 3);
-            const int y = c.brick_xyz[3 * b + 1] * 4 + ((l >> 2) & 3);
-            const int z = c.brick_xyz[3 * b + 2] * 4 + (l >> 4);
-            lab[size_t(b) * kLanes + l] = (x < X && y < Y && z < Z) ? L.map[L.at(x, y, z)] : L_OUTSIDE;
+            int x, y, z;
+            c.cell_of(b, l, x, y, z);
+            lab[size_t(b) * kLanes + l] =
+                (x >= 0 && y >= 0 && x < X && y < Y && z < Z) ? L.map[L.at(x, y, z)] : L_OUTSIDE;
         }
     // neighbour bricks of the active bricks
-    std::vector<int32_t> nbr(size_t(std::max(na, 1)) * kQ, ns);
+    std::vector<int32_t> nbr(size_t(std::max(na, 1)) * kQ, ns * kQ * kLanes);
     std::vector<int16_t> org(size_t(std::max(na, 1)) * 3, 0);
     for (int b = 0; b < na; ++b) {
         const int bx = c.brick_xyz[3 * b], by = c.brick_xyz[3 * b + 1], bz = c.brick_xyz[3 * b + 2];
@@ -397,7 +418,8 @@ int setup_device(lbm355_case& c)
             int32_t s = -1;
             if (nx >= 0 && ny >= 0 && nz >= 0 && nx < c.nbx && ny < c.nby && nz < c.nbz)
                 s = c.brick_id[nx + int64_t(c.nbx) * (ny + int64_t(c.nby) * nz)];
-            nbr[size_t(b) * kQ + o] = s < 0 ? ns : s;
+            // element offset of the neighbour brick's population block
+            nbr[size_t(b) * kQ + o] = (s < 0 ? ns : s) * kQ * kLanes;
         }
     }
     // BC slots in (brick, lane) order
@@ -435,9 +457,8 @@ int setup_device(lbm355_case& c)
         for (int l = 0; l < kLanes; ++l) {
             const int lb = lab[size_t(b) * kLanes + l];
             if (!is_bc_label(lb)) continue;
-            const int x = c.brick_xyz[3 * b] * 4 + (l & 3);
-            const int y = c.brick_xyz[3 * b + 1] * 4 + ((l >> 2) & 3);
-            const int z = c.brick_xyz[3 * b + 2] * 4 + (l >> 4);
+            int x, y, z;
+            c.cell_of(b, l, x, y, z);
             c.bc_cell.push_back(b * kLanes + l);
             c.bc_label_h.push_back(uint8_t(lb));
             int src[kQ][3];
@@ -586,10 +607,9 @@ int gather_dense(lbm355_case& c, const void* dev, int nf, int field, double fill
     std::fill(out, out + N, fill);
     for (int b = 0; b < na; ++b)
         for (int l = 0; l < kLanes; ++l) {
-            const int x = c.brick_xyz[3 * b] * 4 + (l & 3);
-            const int y = c.brick_xyz[3 * b + 1] * 4 + ((l >> 2) & 3);
-            const int z = c.brick_xyz[3 * b + 2] * 4 + (l >> 4);
-            if (x >= X || y >= Y || z >= Z) continue;
+            int x, y, z;
+            c.cell_of(b, l, x, y, z);
+            if (x < 0 || y < 0 || x >= X || y >= Y || z >= Z) continue;
             const int64_t i = c.cell_index(x, y, z);
             if (c.lab.map[i] == L_WALL) continue;
             out[i] = double(h[(size_t(b) * nf + field) * kLanes + l]);
@@ -751,38 +771,36 @@ int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_step_ms, doub
 {
     if (!c || n <= 0) return fail(LBM355_EINVAL, "bad argument");
     int rc;
+    const int64_t n1 = (n + 1) / 2, n2 = n - n1;  // back to back, then per-launch events
     if ((rc = set_case_u64(*c, &c->host.macro_step, uint64_t(c->steps + n - 1)))) return rc;
     StepLaunch L = single_launch(*c);
-    const int64_t nk = std::min<int64_t>(n, 512);
+    const int64_t nk = std::max<int64_t>(1, std::min<int64_t>(n2, 512));
     std::vector<hipEvent_t> ev(size_t(2 * nk + 2));
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-    // pass 1: per-launch events around the fused kernel
-    for (int64_t s = 0; s < nk; ++s) {
-        L.rel = c->steps + s;
-        LAUNCH_TRY(launch_step(c->storage, c->model, 1, L, c->stream));
-        HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
-        LAUNCH_TRY(launch_step(c->storage, c->model, 2, L, c->stream));
-        HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
-    }
-    // pass 2: the remaining steps back to back between one event pair
     HIP_TRY(hipEventRecord(ev[2 * nk], c->stream));
-    for (int64_t s = nk; s < n; ++s) {
+    for (int64_t s = 0; s < n1; ++s) {
         L.rel = c->steps + s;
         LAUNCH_TRY(launch_step(c->storage, c->model, 0, L, c->stream));
     }
     HIP_TRY(hipEventRecord(ev[2 * nk + 1], c->stream));
-    HIP_TRY(hipEventSynchronize(ev[2 * nk + 1]));
+    for (int64_t s = 0; s < n2; ++s) {
+        L.rel = c->steps + n1 + s;
+        if (s < nk) HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
+        LAUNCH_TRY(launch_step(c->storage, c->model, 0, L, c->stream));
+        if (s < nk) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
     double ksum = 0.0;
-    for (int64_t s = 0; s < nk; ++s) {
+    for (int64_t s = 0; s < std::min(nk, n2); ++s) {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]));
         ksum += ms;
     }
-    float rest = 0;
-    HIP_TRY(hipEventElapsedTime(&rest, ev[2 * nk], ev[2 * nk + 1]));
+    float whole = 0;
+    HIP_TRY(hipEventElapsedTime(&whole, ev[2 * nk], ev[2 * nk + 1]));
     for (auto& e : ev) (void)hipEventDestroy(e);
-    if (avg_kernel_ms) *avg_kernel_ms = ksum / double(nk);
-    if (avg_step_ms) *avg_step_ms = n > nk ? double(rest) / double(n - nk) : ksum / double(nk);
+    if (avg_kernel_ms) *avg_kernel_ms = n2 > 0 ? ksum / double(std::min(nk, n2)) : double(whole) / double(n1);
+    if (avg_step_ms) *avg_step_ms = double(whole) / double(n1);
     if (c->d.stats_every > 0) {
         const int64_t k = c->d.stats_every;
         c->stats_steps += (c->steps + n) / k - c->steps / k;

@@ -48,7 +48,7 @@ struct CaseDev {
     void* macro;              // [active brick][4][64] rho,u (compute type), or null
     void* strain;             // [active brick][6][64] strain (compute type), or null
     void* force;              // [active brick][3][64] body force (fd-force), or null
-    const int32_t* nbr;       // [active brick][27] storage index of neighbour bricks
+    const int32_t* nbr;       // [active brick][27] element offset (brick * 27 * 64) of the neighbour bricks
     const uint8_t* label;     // [stored brick + dummy][64]
     const int32_t* bc_base;   // [active brick] first BC slot of the brick or -1
     void* bc_val;             // [nbc][4] rho, ux, uy, uz (compute type)

@@ -489,6 +489,10 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
                                       (C.macro_every > 0 && next % uint32_t(C.macro_every) == 0u));
     const rsrc_t SR = make_rsrc(C.stats, C.stats_bytes);
     R* const mo = uniform(static_cast<R*>(C.macro));
+    // pulsatile inflow factor of this step (wave-uniform)
+    double pulse = 1.0;
+    if (C.pulse_amp != 0.0 && C.pulse_period > 0.0)
+        pulse = 1.0 + C.pulse_amp * sin(6.283185307179586 * double(step) / C.pulse_period);
     const int lab = ((const __attribute__((address_space(1))) uint8_t*)C.label)[b * kLanes + lane];
     const bool fluid = lab == L_FLUID;
     const bool bc = is_bc_label(lab);
@@ -502,7 +506,7 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
     for (int q = 0; q < 27; ++q) t[q] = nbr_c[q];
 
     const int i = lane & 3, j = (lane >> 2) & 3, k = lane >> 4;
-    const uint32_t own = uint32_t(b * 27) * kLanes + uint32_t(lane);
+    const uint32_t own = uint32_t(t[13]) + uint32_t(lane);
 
     R f[27];
     R rho = R(1), ux = R(0), uy = R(0), uz = R(0);
@@ -518,7 +522,7 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
                 const int si = i - cx, sj = j - cy, sk = k - cz;
                 const int nb = pick_brick(t, -cx, -cy, -cz, (si & ~3) != 0, (sj & ~3) != 0, (sk & ~3) != 0);
                 const int l2 = (si & 3) | ((sj & 3) << 2) | ((sk & 3) << 4);
-                f[q] = P::ld(F, uint32_t(nb * 27 + (26 - q)) * kLanes + uint32_t(l2), q);
+                f[q] = P::ld(F, uint32_t(nb + (26 - q) * kLanes + l2), q);
             }
         }
         R Fx = R(0), Fy = R(0), Fz = R(0);
@@ -551,10 +555,33 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
         else if constexpr (MODEL == M_MRT) collide_mrt<R>(f, rho, ux, uy, uz, nu);
         else collide_kbc<R>(f, rho, ux, uy, uz, Fx, Fy, Fz, nu);
     } else if (bc) {
-        // values prepared by the BC pre-pass for this step
+        // sim_tcpc.cu:242-273: the populations entering the partner cell one
+        // plane inwards give rho (inflow) or u (outflow); the partner and any
+        // clamped-edge writer of those slots are lanes of this wavefront (see
+        // the brick origin shift), so these loads precede their stores
         const int slot = C.bc_base[b] + __popcll(bc_lanes & lanes_below(lane));
-        const R* bv = static_cast<const R*>(C.bc_val) + 4 * slot;
-        rho = bv[0]; ux = bv[1]; uy = bv[2]; uz = bv[3];
+        const int32_t* rd = C.bc_read + (slot * 2 + parity) * 27;
+        R g[27];
+#pragma unroll
+        for (int q = 0; q < 27; ++q) g[q] = P::ld(F, uint32_t(rd[q]), q);
+        R s = g[13];
+#pragma unroll
+        for (int q = 0; q < 13; ++q) s += g[q] + g[26 - q];
+        if (is_inflow_label(lab)) {
+            const R base = lab == L_IN_XM ? R(C.inflow_xm) : -R(C.inflow_xp);
+            const R fac = R(C.bc_factor[slot]);
+            ux = (C.plug ? base / fac : base * fac) * R(pulse);
+            rho = s;
+        } else {
+            R mx = R(0), my = R(0), mz = R(0);
+#pragma unroll
+            for (int q = 0; q < 13; ++q) {
+                const R d = g[26 - q] - g[q];
+                mx -= R(qx(q)) * d; my -= R(qy(q)) * d; mz -= R(qz(q)) * d;
+            }
+            ux = mx / s; uy = my / s; uz = mz / s;
+            rho = R(1);
+        }
         feq_boundary<R>(f, rho, ux, uy, uz);
         wmask = C.bc_wmask[slot];
     }
@@ -571,7 +598,7 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
                 const int l2 = (di & 3) | ((dj & 3) << 2) | ((dk & 3) << 4);
                 // BC cells skip the writes that would leave the grid; the
                 // buffer resource drops them anyway, this keeps them off the bus
-                if ((wmask >> q) & 1u) P::st(F, uint32_t(nb * 27 + q) * kLanes + uint32_t(l2), q, f[q]);
+                if ((wmask >> q) & 1u) P::st(F, uint32_t(nb + q * kLanes + l2), q, f[q]);
             }
         }
         // macroscopic output (MacroTcpc::outputMacro, sim_tcpc.cu:337-354)
@@ -621,52 +648,6 @@ __global__ void __launch_bounds__(512) k_step(StepLaunch L)
     }
 }
 
-// BC pre-pass: rho (inflow) or u (outflow) from the populations that enter
-// the partner cell one plane inwards (sim_tcpc.cu:242-273).  A separate launch
-// because, under AA, the partner rewrites exactly those slots in the step.
-template <int S>
-__global__ void __launch_bounds__(256) k_bc_prepass(StepLaunch L)
-{
-    using P = Pop<S>;
-    using R = typename P::R;
-    const int t = int(blockIdx.x * blockDim.x + threadIdx.x);
-    if (t >= L.n_bc_list) return;
-    const uint2 e = L.bc_list[t];
-    if (e.y == 0xFFFFFFFFu) return;  // padding: each wavefront serves one case
-    const CaseDev& C = L.cases[e.x];
-    const int s = int(e.y);
-    const int64_t step = *L.base + L.rel - C.start;
-    const int parity = int(step & 1);
-    const rsrc_t F = make_rsrc(C.f, C.f_bytes);
-    const int32_t* rd = C.bc_read + (s * 2 + parity) * 27;
-    R g[27];
-#pragma unroll
-    for (int q = 0; q < 27; ++q) g[q] = P::ld(F, uint32_t(rd[q]), q);
-    R rho = g[13];
-#pragma unroll
-    for (int q = 0; q < 13; ++q) rho += g[q] + g[26 - q];
-    const int lab = C.bc_label[s];
-    R out[4];
-    if (is_inflow_label(lab)) {
-        R pulse = R(1);
-        if (C.pulse_amp != 0.0 && C.pulse_period > 0.0)
-            pulse = R(1.0 + C.pulse_amp * sin(6.283185307179586 * double(step) / C.pulse_period));
-        const R base = lab == L_IN_XM ? R(C.inflow_xm) : -R(C.inflow_xp);
-        const R u = (C.plug ? base / R(C.bc_factor[s]) : base * R(C.bc_factor[s])) * pulse;
-        out[0] = rho; out[1] = u; out[2] = R(0); out[3] = R(0);
-    } else {
-        R mx = R(0), my = R(0), mz = R(0);
-#pragma unroll
-        for (int q = 0; q < 13; ++q) {
-            const R d = g[26 - q] - g[q];
-            mx -= R(qx(q)) * d; my -= R(qy(q)) * d; mz -= R(qz(q)) * d;
-        }
-        out[0] = R(1); out[1] = mx / rho; out[2] = my / rho; out[3] = mz / rho;
-    }
-    R* bv = static_cast<R*>(C.bc_val) + 4 * s;
-    bv[0] = out[0]; bv[1] = out[1]; bv[2] = out[2]; bv[3] = out[3];
-}
-
 template <int S>
 __global__ void k_fill_rest(typename Pop<S>::T* f, int64_t n)
 {
@@ -710,7 +691,7 @@ __global__ void __launch_bounds__(64) k_strain(const CaseDev* Cp)
         const int ox = (gx < 0) ? -1 : (gx > 3 ? 1 : 0);
         const int oy = (gy < 0) ? -1 : (gy > 3 ? 1 : 0);
         const int oz = (gz < 0) ? -1 : (gz > 3 ? 1 : 0);
-        const int nb = C.nbr[b * 27 + 13 + 9 * ox + 3 * oy + oz];
+        const int nb = C.nbr[b * 27 + 13 + 9 * ox + 3 * oy + oz] / (27 * kLanes);
         const int l = (gx & 3) | ((gy & 3) << 2) | ((gz & 3) << 4);
         const bool fl = C.label[nb * kLanes + l] == L_FLUID;
         sf[c] = fl ? 1 : 0;
@@ -808,11 +789,7 @@ static int check_launch()
 template <int S, int MODEL>
 static int launch_step_t(const StepLaunch& L, int phase, hipStream_t st)
 {
-    if (phase != 2 && L.n_bc_list > 0) {
-        const int nb = (L.n_bc_list + 255) / 256;
-        hipLaunchKernelGGL((k_bc_prepass<S>), dim3(nb), dim3(256), 0, st, L);
-    }
-    if (phase == 1) return check_launch();
+    if (phase == 1) return check_launch();  // the BC reads are part of the fused kernel
     const int bpg = L.bricks_per_group;
     int groups = (L.n_list + bpg - 1) / bpg;
     groups = ((groups + 7) / 8) * 8;
