@@ -82,8 +82,9 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; cases shard across GPUs as independent runs "
-              "(see DESIGN.md), so --gpus must be 1", file=sys.stderr)
+        print("bench.py: only the one-GPU measurement exists; independent cases spread over GPUs "
+              "as separate processes and one case's slab decomposition is not built, so --gpus must be 1",
+              file=sys.stderr)
         return 2
 
     from nonnewlbm_amd import _abi

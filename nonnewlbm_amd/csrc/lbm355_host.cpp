@@ -200,11 +200,11 @@ struct lbm355_case {
     int64_t steps = 0;
     int64_t macro_valid_step = -1;      // step index whose rho,u are in the macro buffer
     int64_t stats_steps = 0;            // steps accumulated since the last reset
-    int64_t stats_first = 0;
+    int64_t stats_first = 0;            // step index of the last reset (the epoch)
     bool strain_valid = false;
 
     hipStream_t stream = nullptr;
-    DevBuf f, stats, macro, strain, force, nbr, label, bc_base, bc_val, bc_read, bc_wmask, bc_factor, bc_label,
+    DevBuf f, stats, stats_part, macro, strain, force, nbr, label, bc_base, bc_cell_d, bc_read, bc_wmask, bc_factor, bc_label,
         origin, base, devcase, list, bc_list, flux;
     CaseDev host{};
     int bricks_per_group = 4;
@@ -503,11 +503,13 @@ int setup_device(lbm355_case& c)
     if ((rc = upload(c.bc_wmask, bc_wmask))) return rc;
     if ((rc = upload(c.bc_factor, bc_factor))) return rc;
     if ((rc = upload(c.bc_label, c.bc_label_h))) return rc;
-    if ((rc = alloc_zero(c, c.bc_val, size_t(std::max(c.n_bc, 1)) * 4 * cb))) return rc;
+    if ((rc = upload(c.bc_cell_d, c.bc_cell))) return rc;
     // rho = 1, u = 0 for the initial macro state
     if ((rc = alloc_zero(c, c.macro, size_t(std::max(na, 1)) * kMacroFields * kLanes * cb))) return rc;
     if ((rc = alloc_zero(c, c.strain, size_t(std::max(na, 1)) * 6 * kLanes * cb))) return rc;
     if (d.stats_every > 0 && (rc = alloc_zero(c, c.stats, size_t(std::max(na, 1)) * kStatFields * kLanes * 8))) return rc;
+    if (d.stats_every > 0 && (rc = alloc_zero(c, c.stats_part, size_t(std::max(na, 1)) * kStatFields * kLanes * 4)))
+        return rc;
     if ((rc = alloc_zero(c, c.base, 8))) return rc;
     if ((rc = alloc_zero(c, c.flux, 4 * 8))) return rc;
     std::vector<BrickRef> list(size_t(std::max(na, 1)));
@@ -521,13 +523,15 @@ int setup_device(lbm355_case& c)
     std::memset(&h, 0, sizeof(h));
     h.f = c.f.p;
     h.stats = static_cast<double*>(d.stats_every > 0 ? c.stats.p : nullptr);
+    h.stats_part = static_cast<float*>(d.stats_every > 0 ? c.stats_part.p : nullptr);
+    h.stats_epoch = 0;
     h.macro = c.macro.p;
     h.strain = c.strain.p;
     h.force = nullptr;
     h.nbr = static_cast<const int32_t*>(c.nbr.p);
     h.label = static_cast<const uint8_t*>(c.label.p);
     h.bc_base = static_cast<const int32_t*>(c.bc_base.p);
-    h.bc_val = c.bc_val.p;
+    h.bc_cell = static_cast<const int32_t*>(c.bc_cell_d.p);
     h.bc_read = static_cast<const int32_t*>(c.bc_read.p);
     h.bc_wmask = static_cast<const uint32_t*>(c.bc_wmask.p);
     h.bc_factor = static_cast<const double*>(c.bc_factor.p);
@@ -717,8 +721,8 @@ int lbm355_case_info_get(const lbm355_case* c, lbm355_case_info* o)
     o->bricks_active = c->n_active;
     o->bricks_stored = c->n_stored;
     int64_t bytes = 0;
-    for (const DevBuf* b : {&c->f, &c->stats, &c->macro, &c->strain, &c->force, &c->nbr, &c->label, &c->bc_base,
-                            &c->bc_val, &c->bc_read, &c->bc_wmask, &c->bc_factor, &c->bc_label, &c->origin,
+    for (const DevBuf* b : {&c->f, &c->stats, &c->stats_part, &c->macro, &c->strain, &c->force, &c->nbr, &c->label, &c->bc_base,
+                            &c->bc_cell_d, &c->bc_read, &c->bc_wmask, &c->bc_factor, &c->bc_label, &c->origin,
                             &c->base, &c->devcase, &c->list, &c->bc_list, &c->flux})
         bytes += int64_t(b->bytes);
     o->device_bytes = bytes;
@@ -870,7 +874,16 @@ int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes)
     }
     if (field >= LBM355_FIELD_SUM_RUX && field <= LBM355_FIELD_SUM_RRUZZ) {
         if (c->d.stats_every <= 0) { std::fill(host, host + N, 0.0); return LBM355_OK; }
-        return gather_dense<double>(*c, c->stats.p, kStatFields, field - LBM355_FIELD_SUM_RUX, 0.0, host);
+        if ((rc = gather_dense<double>(*c, c->stats.p, kStatFields, field - LBM355_FIELD_SUM_RUX, 0.0, host))) return rc;
+        if (!f64 && c->stats_steps % kStatBlock != 0) {
+            // samples of the open FP32 block are not in the FP64 sums yet
+            std::vector<double> part(static_cast<size_t>(N));
+            if ((rc = gather_dense<float>(*c, c->stats_part.p, kStatFields, field - LBM355_FIELD_SUM_RUX, 0.0,
+                                          part.data())))
+                return rc;
+            for (int64_t i = 0; i < N; ++i) host[i] += part[size_t(i)];
+        }
+        return LBM355_OK;
     }
     if (field >= LBM355_FIELD_S11 && field <= LBM355_FIELD_S33) {
         return f64 ? gather_dense<double>(*c, c->strain.p, 6, field - LBM355_FIELD_S11, 0.0, host)
@@ -965,6 +978,12 @@ int lbm355_case_flux(lbm355_case* c, int mean, double out[4])
     std::vector<double> h(size_t(std::max(na, 1)) * kStatFields * kLanes);
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (na > 0) HIP_TRY(hipMemcpy(h.data(), c->stats.p, h.size() * 8, hipMemcpyDeviceToHost));
+    if (c->storage != LBM355_F64 && c->stats_steps % kStatBlock != 0) {
+        // the open FP32 block of samples
+        std::vector<float> part(h.size());
+        if (na > 0) HIP_TRY(hipMemcpy(part.data(), c->stats_part.p, part.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < h.size(); ++i) h[i] += double(part[i]);
+    }
     const double diff = std::max<double>(1.0, double(c->stats_steps));
     for (int p = 0; p < 4; ++p) out[p] = 0.0;
     for (int s = 0; s < c->n_bc; ++s) {
@@ -988,8 +1007,11 @@ int lbm355_case_flux(lbm355_case* c, int mean, double out[4])
 int lbm355_case_reset_stats(lbm355_case* c)
 {
     if (!c) return fail(LBM355_EINVAL, "null case");
-    if (c->d.stats_every > 0)
+    if (c->d.stats_every > 0) {
         LAUNCH_TRY(launch_zero(c->stats.p, int64_t(c->stats.bytes), c->stream));
+        int rc;
+        if ((rc = set_case_u64(*c, &c->host.stats_epoch, uint64_t(c->steps)))) return rc;
+    }
     c->stats_steps = 0;
     c->stats_first = c->steps;
     return LBM355_OK;
@@ -1117,6 +1139,7 @@ struct lbm355_batch {
     int64_t launches = 0;     // steps done by the batch
     int64_t base_host = 0;    // mirror of the device launch base
     std::vector<int64_t> start;  // per case: launch index of its step 0
+    std::vector<int64_t> epoch;  // per case: statistics epoch in the batch copy
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
@@ -1175,6 +1198,7 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
         cd.push_back(c->host);
         cd.back().start = -c->steps;  // batch launch 0 is the case's step c->steps
         b->start.push_back(-c->steps);
+        b->epoch.push_back(c->stats_first);
         for (int k = 0; k < c->n_active; ++k) list.push_back(BrickRef{uint32_t(i), uint32_t(k)});
         // each wavefront of the BC pre-pass serves one case (its buffer
         // descriptors are wave-uniform): pad every case's run to 64 entries
@@ -1221,6 +1245,10 @@ int lbm355_batch_step(lbm355_batch* b, int64_t n)
         if (want != b->start[i]) {
             LAUNCH_TRY(launch_set_u64(&dc->start, uint64_t(want), b->stream));
             b->start[i] = want;
+        }
+        if (b->epoch[i] != b->cases[i]->stats_first) {
+            LAUNCH_TRY(launch_set_u64(&dc->stats_epoch, uint64_t(b->cases[i]->stats_first), b->stream));
+            b->epoch[i] = b->cases[i]->stats_first;
         }
         // rho,u of every case at the last step of this call
         LAUNCH_TRY(launch_set_u64(&dc->macro_step, uint64_t(b->cases[i]->steps + n - 1), b->stream));

@@ -28,6 +28,7 @@ constexpr int kLanes = 64;           // cells per brick = wavefront width
 constexpr int kQ = 27;               // D3Q27
 constexpr int kStatFields = 6;       // sum rho u (3), sum (rho u)^2 (3)
 constexpr int kMacroFields = 4;      // rho, ux, uy, uz
+constexpr int kStatBlock = 16;       // FP32 partial samples per FP64 flush
 
 // cell labels (the reference map values, sim_tcpc.cu:194-203, plus 7/8 for
 // outflow planes on x faces) and 255 = outside the grid
@@ -44,14 +45,15 @@ __host__ __device__ constexpr bool is_inflow_label(int l) { return l == L_IN_XM 
 // requests between replays)
 struct CaseDev {
     void* f;                  // populations, storage type of the launch
-    double* stats;            // [active brick][6][64] statistics sums, or null
+    double* stats;            // [active brick][6][64] FP64 statistics sums, or null
+    float* stats_part;        // [active brick][6][64] FP32 partial sums of the open block
     void* macro;              // [active brick][4][64] rho,u (compute type), or null
     void* strain;             // [active brick][6][64] strain (compute type), or null
     void* force;              // [active brick][3][64] body force (fd-force), or null
     const int32_t* nbr;       // [active brick][27] element offset (brick * 27 * 64) of the neighbour bricks
     const uint8_t* label;     // [stored brick + dummy][64]
     const int32_t* bc_base;   // [active brick] first BC slot of the brick or -1
-    void* bc_val;             // [nbc][4] rho, ux, uy, uz (compute type)
+    const int32_t* bc_cell;   // [nbc] active brick * 64 + lane of each BC slot
     const int32_t* bc_read;   // [nbc][2][27] element offsets of the partner reads
     const uint32_t* bc_wmask; // [nbc] 27-bit mask of the odd-step writes inside the grid
     const double* bc_factor;  // [nbc] parabolic profile factor, or the inlet area (plug)
@@ -60,6 +62,7 @@ struct CaseDev {
     int64_t start;            // launch index of the case's step 0: step = launch - start
     int64_t f_bytes;          // size of the population buffer (< 4 GiB)
     int64_t stats_bytes;      // size of the statistics buffer (< 4 GiB)
+    int64_t stats_epoch;      // step index of the last statistics reset
     int32_t n_active;         // active bricks
     int32_t n_stored;         // stored bricks (dummy brick index = n_stored)
     int32_t n_bc;

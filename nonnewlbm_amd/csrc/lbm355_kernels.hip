@@ -195,38 +195,75 @@ __device__ __forceinline__ void from_central(R& a, R& b, R& c, R u)
     c = fp;
 }
 
+// Moments of a population set up to second order: density, raw momentum
+// J = sum c f and raw second moments P_ab = sum c_a c_b f, summed over the 13
+// opposite pairs (q, 26-q).  The integer lattice coefficients are resolved at
+// compile time (after unrolling), so no multiply by 0 or +-1 is issued.
+template <class R>
+struct Moments {
+    R rho, jx, jy, jz, pxx, pyy, pzz, pxy, pxz, pyz;
+};
+
+template <class R>
+__device__ __forceinline__ Moments<R> moments2(const R (&f)[27])
+{
+    Moments<R> m;
+    R rho = f[13], jx = R(0), jy = R(0), jz = R(0);
+    R pxx = R(0), pyy = R(0), pzz = R(0), pxy = R(0), pxz = R(0), pyz = R(0);
+#pragma unroll
+    for (int q = 0; q < 13; ++q) {
+        const int cx = qx(q), cy = qy(q), cz = qz(q);
+        const R sum = f[q] + f[26 - q];
+        const R dif = f[q] - f[26 - q];  // direction q minus direction -c_q
+        rho += sum;
+        if (cx > 0) jx += dif; else if (cx < 0) jx -= dif;
+        if (cy > 0) jy += dif; else if (cy < 0) jy -= dif;
+        if (cz > 0) jz += dif; else if (cz < 0) jz -= dif;
+        if (cx != 0) pxx += sum;
+        if (cy != 0) pyy += sum;
+        if (cz != 0) pzz += sum;
+        if (cx * cy > 0) pxy += sum; else if (cx * cy < 0) pxy -= sum;
+        if (cx * cz > 0) pxz += sum; else if (cx * cz < 0) pxz -= sum;
+        if (cy * cz > 0) pyz += sum; else if (cy * cz < 0) pyz -= sum;
+    }
+    m.rho = rho; m.jx = jx; m.jy = jy; m.jz = jz;
+    m.pxx = pxx; m.pyy = pyy; m.pzz = pzz; m.pxy = pxy; m.pxz = pxz; m.pyz = pyz;
+    return m;
+}
+
 // index of the central moment kappa_{abc} (orders along x, y, z)
 __device__ __forceinline__ constexpr int K(int a, int b, int c) { return 9 * a + 3 * b + c; }
 
 // Cumulant collision of lbm_col_cum.h (default build: omega1 = 1/(3 nu + 1/2),
-// every other rate 1, so the 3rd..6th order cumulants relax to zero and the
-// post-collision higher central moments are products of 2nd-order ones).
-// The first central moments are mirrored, which applies the force
-// (lbm_col_cum.h:313-315).
+// every other rate 1).  With those rates the post-collision state depends on
+// the pre-collision populations only through rho, the first central moments
+// (mirrored, which applies the force, lbm_col_cum.h:313-315) and the second
+// central moments: 3rd-order cumulants go to 0 and the 4th/6th-order central
+// moments become products of the relaxed 2nd-order ones (lbm_col_cum.h:289-307).
+// The second central moments about u follow from the raw moments as
+// kappa_ab = P_ab - u_a J_b - u_b J_a + rho u_a u_b, which replaces the
+// forward chimera transform; the backward chimera (lbm_col_cum.h:319-415)
+// rebuilds the 27 populations.
 template <class R>
-__device__ __forceinline__ void collide_cumulant(R (&m)[27], R rho, R ux, R uy, R uz, R om)
+__device__ __forceinline__ void collide_cumulant(R (&m)[27], const Moments<R>& M, R ux, R uy, R uz, R om)
 {
-#pragma unroll
-    for (int a = 0; a < 9; ++a) to_central(m[3 * a], m[3 * a + 1], m[3 * a + 2], uz);
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) to_central(m[9 * a + c], m[9 * a + 3 + c], m[9 * a + 6 + c], uy);
-#pragma unroll
-    for (int b = 0; b < 9; ++b) to_central(m[b], m[9 + b], m[18 + b], ux);
-
+    const R rho = M.rho;
     const R inv = Fm<R>::rcp(rho);
     const R om1 = R(1) - om;
-    const R k000 = m[K(0, 0, 0)];
-    const R c110 = om1 * m[K(1, 1, 0)];
-    const R c101 = om1 * m[K(1, 0, 1)];
-    const R c011 = om1 * m[K(0, 1, 1)];
-    const R dxy = om1 * (m[K(2, 0, 0)] - m[K(0, 2, 0)]);
-    const R dxz = om1 * (m[K(2, 0, 0)] - m[K(0, 0, 2)]);
+    const R k100 = M.jx - ux * rho, k010 = M.jy - uy * rho, k001 = M.jz - uz * rho;
+    const R k200 = M.pxx - ux * (R(2) * M.jx - ux * rho);
+    const R k020 = M.pyy - uy * (R(2) * M.jy - uy * rho);
+    const R k002 = M.pzz - uz * (R(2) * M.jz - uz * rho);
+    const R k110 = M.pxy - ux * M.jy - uy * M.jx + rho * ux * uy;
+    const R k101 = M.pxz - ux * M.jz - uz * M.jx + rho * ux * uz;
+    const R k011 = M.pyz - uy * M.jz - uz * M.jy + rho * uy * uz;
+    const R c110 = om1 * k110, c101 = om1 * k101, c011 = om1 * k011;
+    const R dxy = om1 * (k200 - k020);
+    const R dxz = om1 * (k200 - k002);
     const R third = R(1) / R(3);
-    const R c200 = (dxy + dxz + k000) * third;
-    const R c020 = (dxz - R(2) * dxy + k000) * third;
-    const R c002 = (dxy - R(2) * dxz + k000) * third;
+    const R c200 = (dxy + dxz + rho) * third;
+    const R c020 = (dxz - R(2) * dxy + rho) * third;
+    const R c002 = (dxy - R(2) * dxz + rho) * third;
     const R k211 = (c200 * c011 + R(2) * c101 * c110) * inv;
     const R k121 = (c020 * c101 + R(2) * c110 * c011) * inv;
     const R k112 = (c002 * c110 + R(2) * c011 * c101) * inv;
@@ -237,21 +274,41 @@ __device__ __forceinline__ void collide_cumulant(R (&m)[27], R rho, R ux, R uy, 
                    (R(16) * c110 * c101 * c011 + R(4) * (c101 * c101 * c020 + c011 * c011 * c200 + c110 * c110 * c002) +
                     R(2) * c200 * c020 * c002) * (inv * inv);
 
-    m[K(1, 0, 0)] = -m[K(1, 0, 0)];
-    m[K(0, 1, 0)] = -m[K(0, 1, 0)];
-    m[K(0, 0, 1)] = -m[K(0, 0, 1)];
-    m[K(1, 1, 0)] = c110; m[K(1, 0, 1)] = c101; m[K(0, 1, 1)] = c011;
-    m[K(2, 0, 0)] = c200; m[K(0, 2, 0)] = c020; m[K(0, 0, 2)] = c002;
-    m[K(1, 1, 1)] = R(0);
-    m[K(2, 1, 0)] = R(0); m[K(2, 0, 1)] = R(0); m[K(1, 2, 0)] = R(0);
-    m[K(0, 2, 1)] = R(0); m[K(1, 0, 2)] = R(0); m[K(0, 1, 2)] = R(0);
-    m[K(2, 1, 1)] = k211; m[K(1, 2, 1)] = k121; m[K(1, 1, 2)] = k112;
-    m[K(2, 2, 0)] = k220; m[K(0, 2, 2)] = k022; m[K(2, 0, 2)] = k202;
-    m[K(2, 2, 1)] = R(0); m[K(2, 1, 2)] = R(0); m[K(1, 2, 2)] = R(0);
-    m[K(2, 2, 2)] = k222;
-
-#pragma unroll
-    for (int b = 0; b < 9; ++b) from_central(m[b], m[9 + b], m[18 + b], ux);
+    // backward along x for the 9 (y,z)-moment columns; the 3rd and 5th order
+    // central moments are zero, which the per-column code below drops
+    const R a0 = R(0.5) * (ux * ux - ux), a1 = ux - R(0.5);  // f- = a0 k0 + a1 k1 + k2/2
+    auto bx = [&](int col, R k0, R k1, R k2) {
+        const R fm = a0 * k0 + a1 * k1 + R(0.5) * k2;
+        const R fp = fm + ux * k0 + k1;
+        m[col] = fm;
+        m[9 + col] = k0 - fm - fp;
+        m[18 + col] = fp;
+    };
+    auto bx0 = [&](int col, R k0, R k2) {  // k1 = 0
+        const R fm = a0 * k0 + R(0.5) * k2;
+        const R fp = fm + ux * k0;
+        m[col] = fm;
+        m[9 + col] = k0 - fm - fp;
+        m[18 + col] = fp;
+    };
+    auto bx1 = [&](int col, R k1) {  // k0 = k2 = 0
+        const R fm = a1 * k1;
+        const R fp = fm + k1;
+        m[col] = fm;
+        m[9 + col] = -fm - fp;
+        m[18 + col] = fp;
+    };
+    // columns indexed by (b,c) = y,z orders: col = 3b + c
+    bx(K(0, 0, 0), rho, -k100, c200);
+    bx(K(0, 1, 0), -k010, c110, R(0));
+    bx0(K(0, 2, 0), c020, k220);
+    bx(K(0, 0, 1), -k001, c101, R(0));
+    bx0(K(0, 1, 1), c011, k211);
+    bx1(K(0, 2, 1), k121);
+    bx0(K(0, 0, 2), c002, k202);
+    bx1(K(0, 1, 2), k112);
+    bx0(K(0, 2, 2), k022, k222);
+    // then along y and z (general triples)
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -283,6 +340,17 @@ __device__ __forceinline__ void feq_boundary(R (&f)[27], R rho, R ux, R uy, R uz
     f[18] *= rho;
 }
 
+// c_q . F with the lattice coefficients resolved at compile time
+template <class R>
+__device__ __forceinline__ R dot_c(int q, R Fx, R Fy, R Fz)
+{
+    R v = R(0);
+    if (qx(q) > 0) v += Fx; else if (qx(q) < 0) v -= Fx;
+    if (qy(q) > 0) v += Fy; else if (qy(q) < 0) v -= Fy;
+    if (qz(q) > 0) v += Fz; else if (qz(q) < 0) v -= Fz;
+    return v;
+}
+
 // improved BGK with product-form equilibrium and source term
 // (lbm_col_bgk.h:14-140)
 template <class R>
@@ -295,23 +363,17 @@ __device__ __forceinline__ void collide_bgk(R (&f)[27], R rho, R ux, R uy, R uz,
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
         const R feq = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
-        const R cF = R(qx(q)) * Fx + R(qy(q)) * Fy + R(qz(q)) * Fz;
-        f[q] += (feq - f[q]) * om + src * (cF - uF) * feq;
+        f[q] += (feq - f[q]) * om + src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
     }
 }
 
 // regularised collision with a Smagorinsky rate, "MRT_LES"
 // (lbm_col_mrt.h:14-77); the force is not applied, as in the reference
 template <class R>
-__device__ __forceinline__ void collide_mrt(R (&f)[27], R rho, R ux, R uy, R uz, R nu)
+__device__ __forceinline__ void collide_mrt(R (&f)[27], const Moments<R>& M, R ux, R uy, R uz, R nu)
 {
-    R Pxx = 0, Pyy = 0, Pzz = 0, Pxy = 0, Pxz = 0, Pyz = 0;
-#pragma unroll
-    for (int q = 0; q < 27; ++q) {
-        const R cx = R(qx(q)), cy = R(qy(q)), cz = R(qz(q));
-        Pxx += cx * cx * f[q]; Pyy += cy * cy * f[q]; Pzz += cz * cz * f[q];
-        Pxy += cx * cy * f[q]; Pxz += cx * cz * f[q]; Pyz += cy * cz * f[q];
-    }
+    const R rho = M.rho;
+    R Pxx = M.pxx, Pyy = M.pyy, Pzz = M.pzz, Pxy = M.pxy, Pxz = M.pxz, Pyz = M.pyz;
     const R third = R(1) / R(3);
     const R nxx = Pxx - rho * (third + ux * ux), nyy = Pyy - rho * (third + uy * uy), nzz = Pzz - rho * (third + uz * uz);
     const R nxy = Pxy - rho * ux * uy, nxz = Pxz - rho * ux * uz, nyz = Pyz - rho * uy * uz;
@@ -325,14 +387,19 @@ __device__ __forceinline__ void collide_mrt(R (&f)[27], R rho, R ux, R uy, R uz,
     for (int q = 0; q < 27; ++q) {
         const int cx = qx(q), cy = qy(q), cz = qz(q);
         const int c2 = cx * cx + cy * cy + cz * cz;
-        const R cPc = R(cx * cx) * Pxx + R(cy * cy) * Pyy + R(cz * cz) * Pzz +
-                      R(2) * (R(cx * cy) * Pxy + R(cx * cz) * Pxz + R(cy * cz) * Pyz);
-        const R cu = R(cx) * ux + R(cy) * uy + R(cz) * uz;
+        R cPc = R(0);
+        if (cx) cPc += Pxx;
+        if (cy) cPc += Pyy;
+        if (cz) cPc += Pzz;
+        if (cx * cy > 0) cPc += R(2) * Pxy; else if (cx * cy < 0) cPc -= R(2) * Pxy;
+        if (cx * cz > 0) cPc += R(2) * Pxz; else if (cx * cz < 0) cPc -= R(2) * Pxz;
+        if (cy * cz > 0) cPc += R(2) * Pyz; else if (cy * cz < 0) cPc -= R(2) * Pyz;
+        const R cu = dot_c(q, ux, uy, uz);
         f[q] = wgt<R>(q) * (rho * (R(2.5) - R(1.5) * R(c2) + R(3) * cu) + R(4.5) * cPc - R(1.5) * tr);
     }
 }
 
-// shear part s_q(D + T + Q) of KBC N4 from raw moments
+// shear part s_q(D + T + Q) of KBC N4 from the non-equilibrium raw moments
 // (lbm_col_kbc_n.h:21-219): M2 = {xx,yy,zz,xy,xz,yz}, M3 = {xyy,xzz,xxy,yzz,xxz,yyz,xyz}
 template <class R>
 __device__ __forceinline__ R kbc_shear(int q, const R (&M2)[6], const R (&M3)[7])
@@ -340,11 +407,12 @@ __device__ __forceinline__ R kbc_shear(int q, const R (&M2)[6], const R (&M3)[7]
     const int cx = qx(q), cy = qy(q), cz = qz(q);
     const int nz = (cx != 0) + (cy != 0) + (cz != 0);
     const R Nxz = M2[0] - M2[2], Nyz = M2[1] - M2[2], Tr = M2[0] + M2[1] + M2[2];
+    const R sixth = R(1) / R(6);
     if (nz == 0) return -Tr;
     if (nz == 1) {
-        if (cx) return (R(2) * Nxz - Nyz) / R(6) + Tr / R(6) - R(cx) * (M3[0] + M3[1]) * R(0.5);
-        if (cy) return (R(2) * Nyz - Nxz) / R(6) + Tr / R(6) - R(cy) * (M3[2] + M3[3]) * R(0.5);
-        return -(Nxz + Nyz) / R(6) + Tr / R(6) - R(cz) * (M3[4] + M3[5]) * R(0.5);
+        if (cx) return (R(2) * Nxz - Nyz) * sixth + Tr * sixth - R(cx) * (M3[0] + M3[1]) * R(0.5);
+        if (cy) return (R(2) * Nyz - Nxz) * sixth + Tr * sixth - R(cy) * (M3[2] + M3[3]) * R(0.5);
+        return -(Nxz + Nyz) * sixth + Tr * sixth - R(cz) * (M3[4] + M3[5]) * R(0.5);
     }
     if (nz == 2) {
         if (!cz) return (R(cx * cy) * M2[3] + R(cx) * M3[0] + R(cy) * M3[2]) * R(0.25);
@@ -357,26 +425,29 @@ __device__ __forceinline__ R kbc_shear(int q, const R (&M2)[6], const R (&M3)[7]
 // entropic KBC N4 (lbm_col_kbc_n.h:1024-1259): f = k + s + h, shear part
 // relaxed with beta = 1/(6 nu + 1), higher part with the entropic gamma
 template <class R>
-__device__ __forceinline__ void collide_kbc(R (&f)[27], R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz, R nu)
+__device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R ux, R uy, R uz, R Fx, R Fy, R Fz, R nu)
 {
+    const R rho = M.rho;
     R px[3], py[3], pz[3];
     phi3(ux, px); phi3(uy, py); phi3(uz, pz);
-    R M2[6] = {0, 0, 0, 0, 0, 0};
-    R M3[7] = {0, 0, 0, 0, 0, 0, 0};
+    // third-order raw moments from the opposite-pair differences
+    R M3[7] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
 #pragma unroll
-    for (int q = 0; q < 27; ++q) {
-        const R cx = R(qx(q)), cy = R(qy(q)), cz = R(qz(q));
-        const R v = f[q];
-        M2[0] += cx * cx * v; M2[1] += cy * cy * v; M2[2] += cz * cz * v;
-        M2[3] += cx * cy * v; M2[4] += cx * cz * v; M2[5] += cy * cz * v;
-        M3[0] += cx * cy * cy * v; M3[1] += cx * cz * cz * v; M3[2] += cx * cx * cy * v;
-        M3[3] += cy * cz * cz * v; M3[4] += cx * cx * cz * v; M3[5] += cy * cy * cz * v;
-        M3[6] += cx * cy * cz * v;
+    for (int q = 0; q < 13; ++q) {
+        const int cx = qx(q), cy = qy(q), cz = qz(q);
+        const R d = f[q] - f[26 - q];
+        if (cx * cy * cy > 0) M3[0] += d; else if (cx * cy * cy < 0) M3[0] -= d;
+        if (cx * cz * cz > 0) M3[1] += d; else if (cx * cz * cz < 0) M3[1] -= d;
+        if (cx * cx * cy > 0) M3[2] += d; else if (cx * cx * cy < 0) M3[2] -= d;
+        if (cy * cz * cz > 0) M3[3] += d; else if (cy * cz * cz < 0) M3[3] -= d;
+        if (cx * cx * cz > 0) M3[4] += d; else if (cx * cx * cz < 0) M3[4] -= d;
+        if (cy * cy * cz > 0) M3[5] += d; else if (cy * cy * cz < 0) M3[5] -= d;
+        if (cx * cy * cz > 0) M3[6] += d; else if (cx * cy * cz < 0) M3[6] -= d;
     }
     const R third = R(1) / R(3);
     // non-equilibrium parts of the moments: s is linear in them
-    M2[0] -= rho * (third + ux * ux); M2[1] -= rho * (third + uy * uy); M2[2] -= rho * (third + uz * uz);
-    M2[3] -= rho * ux * uy; M2[4] -= rho * ux * uz; M2[5] -= rho * uy * uz;
+    R M2[6] = {M.pxx - rho * (third + ux * ux), M.pyy - rho * (third + uy * uy), M.pzz - rho * (third + uz * uz),
+               M.pxy - rho * ux * uy, M.pxz - rho * ux * uz, M.pyz - rho * uy * uz};
     M3[0] -= rho * ux * (third + uy * uy); M3[1] -= rho * ux * (third + uz * uz);
     M3[2] -= rho * uy * (third + ux * ux); M3[3] -= rho * uy * (third + uz * uz);
     M3[4] -= rho * uz * (third + ux * ux); M3[5] -= rho * uz * (third + uy * uy);
@@ -402,53 +473,60 @@ __device__ __forceinline__ void collide_kbc(R (&f)[27], R rho, R ux, R uy, R uz,
         const R feq = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
         const R ds = kbc_shear<R>(q, M2, M3);
         const R dh = f[q] - feq - ds;
-        const R cF = R(qx(q)) * Fx + R(qy(q)) * Fy + R(qz(q)) * Fz;
-        f[q] -= beta * (R(2) * ds + gamma * dh) - src * (cF - uF) * feq;
+        f[q] -= beta * (R(2) * ds + gamma * dh) - src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
     }
 }
 
 // Carreau-Yasuda: nu = nu_ref + (nu0 - nu_ref) (1 + (lambda gamma)^a)^((n-1)/a)
 // (solver/lbm3d/core.h:766, sim_tcpc.cu:697-707)
-//
-// local viscosity of the "neq" coupling: gamma from the deviatoric
-// non-equilibrium second moment P, gamma * tau(gamma) = 3 sqrt(2)|P|/(2 rho),
-// solved by 8 safeguarded Newton steps (the same iteration as the oracle)
 template <class R>
-__device__ __forceinline__ R neq_viscosity(const R (&f)[27], R rho, R ux, R uy, R uz, const CaseDev& C)
+struct Lg;
+template <>
+struct Lg<double> {
+    __device__ static double log2_(double x) { return log2(x); }
+    __device__ static double exp2_(double x) { return exp2(x); }
+};
+template <>
+struct Lg<float> {
+    __device__ static float log2_(float x) { return __builtin_amdgcn_logf(x); }
+    __device__ static float exp2_(float x) { return __builtin_amdgcn_exp2f(x); }
+};
+
+// local viscosity of the "neq" coupling: gamma from the deviatoric
+// non-equilibrium second moment P, gamma * tau(gamma) = G = 3 sqrt(2)|P|/(2 rho).
+// Solved for t = log2(gamma) by 3 Newton steps on H(t) = t + log2(tau) - log2(G)
+// from the middle of the bracket (gamma*tau is close to a power law, so H is
+// nearly linear in t); the oracle runs the same iteration.
+template <class R>
+__device__ __forceinline__ R neq_viscosity(const Moments<R>& M, R ux, R uy, R uz, const CaseDev& C)
 {
-    R Pxx = 0, Pyy = 0, Pzz = 0, Pxy = 0, Pxz = 0, Pyz = 0;
-#pragma unroll
-    for (int q = 0; q < 27; ++q) {
-        const R cx = R(qx(q)), cy = R(qy(q)), cz = R(qz(q));
-        Pxx += cx * cx * f[q]; Pyy += cy * cy * f[q]; Pzz += cz * cz * f[q];
-        Pxy += cx * cy * f[q]; Pxz += cx * cz * f[q]; Pyz += cy * cz * f[q];
-    }
-    Pxx -= rho * ux * ux; Pyy -= rho * uy * uy; Pzz -= rho * uz * uz;
-    Pxy -= rho * ux * uy; Pxz -= rho * ux * uz; Pyz -= rho * uy * uz;
+    const R rho = M.rho;
+    const R Pxx = M.pxx - rho * ux * ux, Pyy = M.pyy - rho * uy * uy, Pzz = M.pzz - rho * uz * uz;
+    const R Pxy = M.pxy - rho * ux * uy, Pxz = M.pxz - rho * ux * uz, Pyz = M.pyz - rho * uy * uz;
     const R tr = (Pxx + Pyy + Pzz) * (R(1) / R(3));
     const R dxx = Pxx - tr, dyy = Pyy - tr, dzz = Pzz - tr;
     const R s2 = dxx * dxx + dyy * dyy + dzz * dzz + R(2) * (Pxy * Pxy + Pxz * Pxz + Pyz * Pyz);
-    const R G = R(3) * Fm<R>::sqrt_(R(2)) * Fm<R>::sqrt_(s2) * Fm<R>::rcp(R(2) * rho);
-    const R nr = R(C.nu_ref), n0 = R(C.cy_nu0), lam = R(C.cy_lambda), a = R(C.cy_a), n = R(C.cy_n);
+    const R G = R(2.1213203435596424) * Fm<R>::sqrt_(s2) * Fm<R>::rcp(rho);  // 3 sqrt(2) / 2
+    const R nr = R(C.nu_ref), n0 = R(C.cy_nu0), a = R(C.cy_a), n = R(C.cy_n);
     const R e = (n - R(1)) / a;
+    if (!(G > R(1e-30))) return n0;
+    const R lG = Lg<R>::log2_(G), L = Lg<R>::log2_(R(C.cy_lambda));
     const R tr_ = R(3) * nr + R(0.5), t0 = R(3) * n0 + R(0.5);
-    R lo = Fm<R>::div(G, fmax(tr_, t0)), hi = Fm<R>::div(G, fmin(tr_, t0));
-    R g = hi;
-#pragma unroll 1
-    for (int it = 0; it < 8; ++it) {
-        const R x = Fm<R>::pow_(lam * g, a);
+    const R lo = lG - Lg<R>::log2_(fmax(tr_, t0)), hi = lG - Lg<R>::log2_(fmin(tr_, t0));
+    R t = R(0.5) * (lo + hi);
+#pragma unroll
+    for (int it = 0; it < 3; ++it) {
+        const R x = Lg<R>::exp2_(a * (L + t));
         const R b = R(1) + x;
-        const R pw = Fm<R>::pow_(b, e);
-        const R nu = nr + (n0 - nr) * pw;
-        const R h = g * (R(3) * nu + R(0.5)) - G;
-        const R dgnu = nr + (n0 - nr) * pw * (R(1) + (n - R(1)) * Fm<R>::div(x, b));
-        const R dh = R(3) * dgnu + R(0.5);
-        if (h > R(0)) hi = g; else lo = g;
-        R gn = g - Fm<R>::div(h, dh);
-        if (!(gn > lo && gn < hi)) gn = R(0.5) * (lo + hi);
-        g = gn;
+        const R pw = Lg<R>::exp2_(e * Lg<R>::log2_(b));
+        const R tau = R(3) * (nr + (n0 - nr) * pw) + R(0.5);
+        const R H = t + Lg<R>::log2_(tau) - lG;
+        const R dnu = (n0 - nr) * pw * (n - R(1)) * Fm<R>::div(x, b);
+        const R Hp = R(1) + R(3) * Fm<R>::div(dnu, tau);
+        t = fmin(fmax(t - Fm<R>::div(H, Hp), lo), hi);
     }
-    return nr + (n0 - nr) * Fm<R>::pow_(R(1) + Fm<R>::pow_(lam * g, a), e);
+    const R x = Lg<R>::exp2_(a * (L + t));
+    return nr + (n0 - nr) * Lg<R>::exp2_(e * Lg<R>::log2_(R(1) + x));
 }
 
 // ---------------------------------------------------------------------------
@@ -483,8 +561,13 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
     const int parity = int(step & 1);
     // per-step output decisions, wave-uniform (32-bit step arithmetic)
     const uint32_t next = uint32_t(step) + 1u;
+    // statistics sample index since the last reset; samples are summed in
+    // FP32 blocks of kStatBlock and each block is added to the FP64 sums
+    const uint32_t snext = uint32_t(step - C.stats_epoch) + 1u;
     const bool do_stats = C.stats && C.stats_every > 0 &&
-                          (C.stats_every == 1 || next % uint32_t(C.stats_every) == 0u);
+                          (C.stats_every == 1 || snext % uint32_t(C.stats_every) == 0u);
+    const uint32_t spos = (C.stats_every == 1 ? snext - 1u : snext / uint32_t(C.stats_every) - 1u) % kStatBlock;
+    const rsrc_t PR = make_rsrc(C.stats_part, C.stats_bytes / 2);
     const bool do_macro = C.macro && (step == C.macro_step ||
                                       (C.macro_every > 0 && next % uint32_t(C.macro_every) == 0u));
     const rsrc_t SR = make_rsrc(C.stats, C.stats_bytes);
@@ -532,28 +615,20 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
             Fy = fo[(b * 3 + 1) * kLanes + lane];
             Fz = fo[(b * 3 + 2) * kLanes + lane];
         }
-        // density in opposite pairs, then momentum (lbm_common.h:16-39)
-        R s = f[13];
-#pragma unroll
-        for (int q = 0; q < 13; ++q) s += f[q] + f[26 - q];
-        rho = s;
-        R mx = R(0), my = R(0), mz = R(0);
-#pragma unroll
-        for (int q = 0; q < 13; ++q) {
-            const R d = f[26 - q] - f[q];  // direction 26-q has c = -c_q
-            mx -= R(qx(q)) * d; my -= R(qy(q)) * d; mz -= R(qz(q)) * d;
-        }
+        // density and momentum (lbm_common.h:16-39) with the second moments
+        const Moments<R> M = moments2<R>(f);
+        rho = M.rho;
         const R ir = Fm<R>::rcp(rho);
-        ux = (mx + R(0.5) * Fx) * ir;
-        uy = (my + R(0.5) * Fy) * ir;
-        uz = (mz + R(0.5) * Fz) * ir;
+        ux = (M.jx + R(0.5) * Fx) * ir;
+        uy = (M.jy + R(0.5) * Fy) * ir;
+        uz = (M.jz + R(0.5) * Fz) * ir;
         R nu = R(C.nu_ref);
-        if (C.cy_mode == 1) nu = neq_viscosity<R>(f, rho, ux, uy, uz, C);
+        if (C.cy_mode == 1) nu = neq_viscosity<R>(M, ux, uy, uz, C);
         const R om = Fm<R>::rcp(R(3) * nu + R(0.5));
-        if constexpr (MODEL == M_CUM) collide_cumulant<R>(f, rho, ux, uy, uz, om);
+        if constexpr (MODEL == M_CUM) collide_cumulant<R>(f, M, ux, uy, uz, om);
         else if constexpr (MODEL == M_BGK) collide_bgk<R>(f, rho, ux, uy, uz, Fx, Fy, Fz, om);
-        else if constexpr (MODEL == M_MRT) collide_mrt<R>(f, rho, ux, uy, uz, nu);
-        else collide_kbc<R>(f, rho, ux, uy, uz, Fx, Fy, Fz, nu);
+        else if constexpr (MODEL == M_MRT) collide_mrt<R>(f, M, ux, uy, uz, nu);
+        else collide_kbc<R>(f, M, ux, uy, uz, Fx, Fy, Fz, nu);
     } else if (bc) {
         // sim_tcpc.cu:242-273: the populations entering the partner cell one
         // plane inwards give rho (inflow) or u (outflow); the partner and any
@@ -564,22 +639,14 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
         R g[27];
 #pragma unroll
         for (int q = 0; q < 27; ++q) g[q] = P::ld(F, uint32_t(rd[q]), q);
-        R s = g[13];
-#pragma unroll
-        for (int q = 0; q < 13; ++q) s += g[q] + g[26 - q];
+        const Moments<R> G = moments2<R>(g);
         if (is_inflow_label(lab)) {
             const R base = lab == L_IN_XM ? R(C.inflow_xm) : -R(C.inflow_xp);
             const R fac = R(C.bc_factor[slot]);
             ux = (C.plug ? base / fac : base * fac) * R(pulse);
-            rho = s;
+            rho = G.rho;
         } else {
-            R mx = R(0), my = R(0), mz = R(0);
-#pragma unroll
-            for (int q = 0; q < 13; ++q) {
-                const R d = g[26 - q] - g[q];
-                mx -= R(qx(q)) * d; my -= R(qy(q)) * d; mz -= R(qz(q)) * d;
-            }
-            ux = mx / s; uy = my / s; uz = mz / s;
+            ux = G.jx / G.rho; uy = G.jy / G.rho; uz = G.jz / G.rho;
             rho = R(1);
         }
         feq_boundary<R>(f, rho, ux, uy, uz);
@@ -609,14 +676,41 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
             mo[(b * 4 + 3) * kLanes + lane] = uz;
         }
         if (do_stats) {
-            const uint32_t o = (uint32_t(b * kStatFields) * kLanes + uint32_t(lane)) * 8u;
-            const double rx = double(rho) * double(ux), ry = double(rho) * double(uy), rz = double(rho) * double(uz);
-            const double add[6] = {rx, ry, rz, rx * rx, ry * ry, rz * rz};
-            double cur[6];
+            const uint32_t idx = uint32_t(b * kStatFields) * kLanes + uint32_t(lane);
+            if constexpr (S == S_F64) {
+                // FP64 runs add every sample to the FP64 sums directly
+                const double rx = rho * ux, ry = rho * uy, rz = rho * uz;
+                const double add[6] = {rx, ry, rz, rx * rx, ry * ry, rz * rz};
+                double cur[6];
 #pragma unroll
-            for (int a = 0; a < 6; ++a) cur[a] = ld_f64(SR, o + uint32_t(a * kLanes * 8));
+                for (int a = 0; a < 6; ++a) cur[a] = ld_f64(SR, (idx + uint32_t(a * kLanes)) * 8u);
 #pragma unroll
-            for (int a = 0; a < 6; ++a) st_f64(SR, o + uint32_t(a * kLanes * 8), cur[a] + add[a]);
+                for (int a = 0; a < 6; ++a) st_f64(SR, (idx + uint32_t(a * kLanes)) * 8u, cur[a] + add[a]);
+            } else {
+                const float rx = float(rho * ux), ry = float(rho * uy), rz = float(rho * uz);
+                const float add[6] = {rx, ry, rz, rx * rx, ry * ry, rz * rz};
+                if (spos == 0u) {
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) st_f32(PR, (idx + uint32_t(a * kLanes)) * 4u, add[a]);
+                } else if (spos < uint32_t(kStatBlock - 1)) {
+                    float cur[6];
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) cur[a] = ld_f32(PR, (idx + uint32_t(a * kLanes)) * 4u);
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) st_f32(PR, (idx + uint32_t(a * kLanes)) * 4u, cur[a] + add[a]);
+                } else {
+                    float part[6];
+                    double cur[6];
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) {
+                        part[a] = ld_f32(PR, (idx + uint32_t(a * kLanes)) * 4u);
+                        cur[a] = ld_f64(SR, (idx + uint32_t(a * kLanes)) * 8u);
+                    }
+#pragma unroll
+                    for (int a = 0; a < 6; ++a)
+                        st_f64(SR, (idx + uint32_t(a * kLanes)) * 8u, cur[a] + double(part[a] + add[a]));
+                }
+            }
         }
     }
 }
@@ -761,7 +855,10 @@ __global__ void __launch_bounds__(256) k_flux(const CaseDev* Cp, double* out)
         case L_OUT_YM: plane = 2; comp = 2; sign = -1.0; break; // outward = -y
         default: plane = 3; comp = 2; sign = 1.0; break;        // outward = +y
         }
-        const double u = double(static_cast<const R*>(C.bc_val)[4 * s + comp]);
+        // velocity of the BC cell from the macro output of the last step
+        const int cell = C.bc_cell[s];
+        const int b = cell >> 6, l = cell & 63;
+        const double u = double(static_cast<const R*>(C.macro)[(b * 4 + comp) * kLanes + l]);
         v[plane] = u * sign;
     }
 #pragma unroll
