@@ -235,10 +235,16 @@ int lbm355_batch_autotune(lbm355_batch* b, int* best_bricks_per_group);
 
 /* bytes of device memory a descriptor would need (admission control) */
 int lbm355_case_footprint(const lbm355_case_desc* desc, int64_t* bytes);
+/* the label setup of lbm355_case_create without a device: labels (nx*ny*nz,
+   x-fastest, may be NULL) receive the final map (0 fluid, 1 wall, 3/4 x-planes,
+   5/6 y-planes, 7/8 x+/x- outflows) and info the counts, areas, brick counts
+   and footprint.  Returns LBM355_ETERMINATE on a zero inlet/outlet area, as
+   setupBoundaries does (sim_tcpc.cu:632-639). */
+int lbm355_case_labels(const lbm355_case_desc* desc, int32_t* labels, lbm355_case_info* info);
 
 /* --- raw kernel timing (measurement only) ---------------------------------- */
 /* n more time steps, timed with HIP events on the case's stream: avg_step_ms =
-   whole step (BC pre-pass + fused kernel) from one event pair around the n
+   whole step (the fused kernel and its launch gaps) from one event pair around the n
    steps; avg_kernel_ms = the fused stream-collide launch alone, from an event
    pair around each launch of the first min(n, 512) steps.  No counterpart in
    the reference, which only logs wall-clock MLUPS (core.h:1440-1469). */

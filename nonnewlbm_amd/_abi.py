@@ -167,6 +167,7 @@ def load() -> C.CDLL:
         "lbm355_batch_set_shape": (C.c_int, [P, C.c_int]),
         "lbm355_batch_autotune": (C.c_int, [P, C.POINTER(C.c_int)]),
         "lbm355_case_footprint": (C.c_int, [C.POINTER(CaseDesc), C.POINTER(C.c_int64)]),
+        "lbm355_case_labels": (C.c_int, [C.POINTER(CaseDesc), C.POINTER(C.c_int32), C.POINTER(CaseInfo)]),
         "lbm355_case_time_kernel": (C.c_int, [P, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "lbm355_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
     }
@@ -178,14 +179,15 @@ def load() -> C.CDLL:
     return lib
 
 
-# every symbol include/lbm355.h declares (checked by tests/test_abi.py)
+# every symbol include/lbm355.h declares
 EXPORTED = [
     "lbm355_version", "lbm355_last_error", "lbm355_device_count",
     "lbm355_case_create", "lbm355_case_destroy", "lbm355_case_info_get", "lbm355_case_set_inflow",
     "lbm355_case_step", "lbm355_case_sync", "lbm355_case_get_field", "lbm355_case_get_populations",
     "lbm355_case_compute_strain", "lbm355_case_flux", "lbm355_case_reset_stats", "lbm355_case_write_vtk",
     "lbm355_batch_create", "lbm355_batch_step", "lbm355_batch_destroy", "lbm355_batch_set_shape",
-    "lbm355_batch_autotune", "lbm355_case_footprint", "lbm355_case_time_kernel", "lbm355_copy_bandwidth",
+    "lbm355_batch_autotune", "lbm355_case_footprint", "lbm355_case_labels", "lbm355_case_time_kernel",
+    "lbm355_copy_bandwidth",
 ]
 
 

@@ -671,6 +671,40 @@ int lbm355_case_footprint(const lbm355_case_desc* desc, int64_t* bytes)
     return LBM355_OK;
 }
 
+int lbm355_case_labels(const lbm355_case_desc* desc, int32_t* labels, lbm355_case_info* o)
+{
+    int rc = validate(desc);
+    if (rc) return rc;
+    lbm355_case c;
+    c.d = *desc;
+    if ((rc = c.lab.build(c.d))) return rc;
+    if ((rc = build_index(c))) return rc;
+    const int64_t N = int64_t(desc->nx) * desc->ny * desc->nz;
+    if (labels)
+        for (int64_t i = 0; i < N; ++i) labels[i] = c.lab.map[i];
+    if (o) {
+        std::memset(o, 0, sizeof(*o));
+        o->fluid = c.lab.n_fluid;
+        o->wall = c.lab.n_wall;
+        o->area_ivc = c.lab.area[0];
+        o->area_svc = c.lab.area[1];
+        o->area_out5 = c.lab.area[2];
+        o->area_out6 = c.lab.area[3];
+        o->input_label3 = c.lab.in_lab[0];
+        o->input_label4 = c.lab.in_lab[1];
+        o->input_label5 = c.lab.in_lab[2];
+        o->input_label6 = c.lab.in_lab[3];
+        o->active_cells = N - c.lab.n_wall;
+        o->bricks_active = c.n_active;
+        o->bricks_stored = c.n_stored;
+        int64_t nbc = 0;
+        for (int i = 0; i < 4; ++i) nbc += c.lab.area[i];
+        o->device_bytes = footprint_bytes(c.d, c.n_active, c.n_stored, nbc);
+        o->terminate = c.lab.terminate ? 1 : 0;
+    }
+    return c.lab.terminate ? fail(LBM355_ETERMINATE, "zero inlet or outlet area") : LBM355_OK;
+}
+
 int lbm355_case_create(const lbm355_case_desc* desc, lbm355_case** out)
 {
     int rc = validate(desc);

@@ -167,3 +167,18 @@ def footprint(geometry_flat: np.ndarray, dims: Sequence[int], **kw) -> int:
     check(load().lbm355_case_footprint(C.byref(d), C.byref(b)), "footprint")
     del keep
     return b.value
+
+
+def labels(geometry_flat: np.ndarray, dims: Sequence[int], **kw) -> tuple[np.ndarray, dict]:
+    """The label setup of a case without a device: (labels[x, y, z], info).
+
+    A zero inlet/outlet area is reported as info["terminate"] = 1 (no raise)."""
+    d, keep = make_desc(geometry_flat, dims, **kw)
+    X, Y, Z = (int(v) for v in dims)
+    out = np.empty(X * Y * Z, dtype=np.int32)
+    info = CaseInfo()
+    rc = load().lbm355_case_labels(C.byref(d), out.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(info))
+    if rc not in (_abi.OK, _abi.ETERMINATE):
+        check(rc, "labels")
+    del keep
+    return out.reshape(Z, Y, X).transpose(2, 1, 0), info.as_dict()

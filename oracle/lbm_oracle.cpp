@@ -7,10 +7,12 @@
 // bench.py's cpu_baseline leg load it.  The product (nonnewlbm_amd/) never does.
 //
 // Each function names the reference lines it restates.  The reference itself
-// cannot be compiled here (TNL, VTK C++ and libpng are absent), so this
-// restatement is pinned by the numpy brute-force checks in
-// tests/test_oracle_pins.py (moments by 27x27 matrices, closed forms,
-// conservation, fixed points) and by the golden vectors in tests/golden/.
+// cannot be compiled here (TNL, VTK C++ and libpng are absent) and ships no
+// golden vectors, so this restatement is pinned by the numpy checks in
+// tests/test_oracle_pins.py: central moments by 27x27 matrices, the Gaussian
+// (Isserlis) closure of the cumulant collision, closed forms, conservation,
+// fixed points, the Chapman-Enskog stress of a known shear rate, and
+// brute-force transport on tiny lattices.
 //
 // Deviations from the reference, all deliberate:
 //  - statistics sums are kept in double for every storage type (the reference
@@ -395,6 +397,9 @@ struct OracleBase {
     virtual int compute_strain() = 0;
     virtual int info(lbm355_case_info* i) = 0;
     virtual int set_inflow(double a, double b) = 0;
+    virtual int set_populations(const double* in) = 0;
+    virtual int stream_only(int64_t n) = 0;
+    virtual int set_velocity(const double* ux, const double* uy, const double* uz) = 0;
 };
 
 template <class T>
@@ -806,6 +811,50 @@ struct Oracle : OracleBase {
         return LBM355_EINVAL;
     }
 
+    // test hooks: overwrite the current populations (layout of get_populations)
+    int set_populations(const double* in) override
+    {
+        std::vector<T>& s = df[cur];
+        for (int q = 0; q < 27; ++q)
+            for (int z = 0; z < Z; ++z)
+                for (int y = 0; y < Y; ++y)
+                    for (int x = 0; x < X; ++x) s[DF(q, x, y, z)] = T(in[q * N + I(x, y, z)]);
+        return 0;
+    }
+
+    // test hook: n steps of streaming alone (pull for fluid, full-way
+    // bounce-back for wall cells, no collision), for lattices with fluid and
+    // wall labels only
+    int stream_only(int64_t n) override
+    {
+        for (int64_t i = 0; i < N; ++i)
+            if (map[i] != GEO_FLUID && map[i] != GEO_WALL) { g_err = "stream_only: boundary labels present"; return LBM355_EINVAL; }
+        for (int64_t s = 0; s < n; ++s) {
+            const std::vector<T>& src = df[cur];
+            std::vector<T>& dst = df[1 - cur];
+            for (int x = 0; x < X; ++x)
+                for (int z = 0; z < Z; ++z)
+                    for (int y = 0; y < Y; ++y) {
+                        T f[27];
+                        const int xp = std::min(x + 1, X - 1), xm = std::max(x - 1, 0);
+                        const int yp = std::min(y + 1, Y - 1), ym = std::max(y - 1, 0);
+                        const int zp = std::min(z + 1, Z - 1), zm = std::max(z - 1, 0);
+                        if (map[I(x, y, z)] == GEO_FLUID) stream(src, f, xm, x, xp, ym, y, yp, zm, z, zp);
+                        else stream_bb(src, f, xm, x, xp, ym, y, yp, zm, z, zp);
+                        for (int q = 0; q < 27; ++q) dst[DF(q, x, y, z)] = f[q];
+                    }
+            cur = 1 - cur;
+        }
+        return 0;
+    }
+
+    // test hook: overwrite the macroscopic velocity (input of compute_strain)
+    int set_velocity(const double* ux, const double* uy, const double* uz) override
+    {
+        for (int64_t i = 0; i < N; ++i) { vx[i] = T(ux[i]); vy[i] = T(uy[i]); vz[i] = T(uz[i]); }
+        return 0;
+    }
+
     int get_populations(double* out) override
     {
         const std::vector<T>& s = df[cur];
@@ -848,8 +897,14 @@ int oracle_get_field(void* o, int field, double* out) { return static_cast<Oracl
 int oracle_get_populations(void* o, double* out) { return static_cast<OracleBase*>(o)->get_populations(out); }
 int oracle_compute_strain(void* o) { return static_cast<OracleBase*>(o)->compute_strain(); }
 int oracle_info(void* o, lbm355_case_info* i) { return static_cast<OracleBase*>(o)->info(i); }
+int oracle_set_populations(void* o, const double* in) { return static_cast<OracleBase*>(o)->set_populations(in); }
+int oracle_stream_only(void* o, int64_t n) { return static_cast<OracleBase*>(o)->stream_only(n); }
+int oracle_set_velocity(void* o, const double* ux, const double* uy, const double* uz)
+{
+    return static_cast<OracleBase*>(o)->set_velocity(ux, uy, uz);
+}
 
-// single-cell entry points for the numpy pins (tests/test_oracle_pins.py)
+// single-cell entry points for the numpy pins
 int oracle_collide(int model, int f64, const double* fin, double rho, const double* u, const double* F,
                    double nu, double* fout)
 {

@@ -6,6 +6,7 @@ cpu_baseline leg of bench.py, always as the checker, never as the product.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import subprocess
 from pathlib import Path
 
@@ -14,7 +15,8 @@ import numpy as np
 from nonnewlbm_amd._abi import FIELD, CaseDesc, CaseInfo, make_desc
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "_build" / "liboracle.so"
+# ORACLE_LIB points the checker at another build of the same source
+LIB = Path(os.environ.get("ORACLE_LIB", HERE / "_build" / "liboracle.so"))
 _lib = None
 
 
@@ -45,6 +47,11 @@ def load() -> C.CDLL:
     lib.oracle_density_velocity.argtypes = [D, D, D]
     lib.oracle_neq_nu.restype = C.c_double
     lib.oracle_neq_nu.argtypes = [D, C.c_double, D, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double]
+    lib.oracle_set_populations.argtypes = [C.c_void_p, D]
+    lib.oracle_stream_only.argtypes = [C.c_void_p, C.c_int64]
+    lib.oracle_set_velocity.argtypes = [C.c_void_p, D, D, D]
+    lib.oracle_neq_nu_from_G.restype = C.c_double
+    lib.oracle_neq_nu_from_G.argtypes = [C.c_double] * 6
     lib.oracle_last_error.restype = C.c_char_p
     _lib = lib
     return lib
@@ -91,6 +98,19 @@ class OracleCase:
         self.lib.oracle_get_populations(self.h, _p(out))
         return out.reshape(27, Z, Y, X).transpose(0, 3, 2, 1)
 
+    def set_populations(self, pops: np.ndarray) -> None:
+        """Overwrite the current populations; ``pops`` is [27, X, Y, Z]."""
+        a = np.ascontiguousarray(np.asarray(pops, dtype=np.float64).transpose(0, 3, 2, 1))
+        self.lib.oracle_set_populations(self.h, _p(a))
+
+    def stream_only(self, n: int = 1) -> None:
+        if self.lib.oracle_stream_only(self.h, int(n)) != 0:
+            raise RuntimeError(self.lib.oracle_last_error().decode())
+
+    def set_velocity(self, ux: np.ndarray, uy: np.ndarray, uz: np.ndarray) -> None:
+        fl = [np.ascontiguousarray(np.asarray(v, dtype=np.float64).transpose(2, 1, 0)) for v in (ux, uy, uz)]
+        self.lib.oracle_set_velocity(self.h, _p(fl[0]), _p(fl[1]), _p(fl[2]))
+
     def compute_strain(self) -> None:
         self.lib.oracle_compute_strain(self.h)
 
@@ -128,3 +148,7 @@ def neq_nu(f: np.ndarray, rho: float, u, nu_ref: float, nu0: float, lam: float, 
     fin = np.ascontiguousarray(f, dtype=np.float64)
     uu = np.asarray(u, dtype=np.float64)
     return float(load().oracle_neq_nu(_p(fin), rho, _p(uu), nu_ref, nu0, lam, a, n))
+
+
+def neq_nu_from_G(G: float, nu_ref: float, nu0: float, lam: float, a: float, n: float) -> float:
+    return float(load().oracle_neq_nu_from_G(G, nu_ref, nu0, lam, a, n))

@@ -1,0 +1,97 @@
+"""The drop-in boundary on the CPU: run_case argv (scripts/run_case.py:19-60),
+the sim_tcpc command line and exit codes (sim_tcpc.cu:70-125,846-857), its
+geometry reader and log lines up to the device, and the flow post-processor
+(postprocess/compute_flows.py:79-147) on arrays with known velocities."""
+from __future__ import annotations
+
+import shlex
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from nonnewlbm_amd import flows, lbm355
+from nonnewlbm_amd.build import EXE
+from nonnewlbm_amd.geometry import tcpc_mask, write_wall_vtk
+from nonnewlbm_amd.run_case import build_command, main as run_case_main
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_run_case_argv_closed_form(capsys):
+    cmd = build_command(ROOT, "v1", "sx")
+    assert cmd[0] == str(ROOT / "nonnewlbm_amd/bin/sim_tcpc")
+    assert cmd[1:] == ["--geometry", str(ROOT / "runtime/geometry.vtk"), "--material", "sx", "--case-id", "v1_sx",
+                       "--ivc-mls", "8.0", "--svc-mls", "4.833333333", "--final-time", "10.0", "--vtk-period", "1.0",
+                       "--print-period", "0.1", "--stat-reset-period", "-1.0", "--lbm-viscosity", "0.0001",
+                       "--block-size", "32"]
+    assert build_command(ROOT, "v4a", "gx", "x", lbm_viscosity=1e-3)[7:11] == ["--ivc-mls", "33.33333333",
+                                                                                "--svc-mls", "18.33333333"]
+    with pytest.raises(ValueError, match="Unknown regime"):
+        build_command(ROOT, "v9", "sx")
+    assert run_case_main(["--regime", "v2", "--material", "newtonian", "--dry-run", "--precision", "fp16"]) == 0
+    out = capsys.readouterr().out.strip()
+    assert shlex.split(out)[-2:] == ["--precision", "fp16"] and "--ivc-mls 16.66666667" in out
+
+
+def _run(args, cwd):
+    return subprocess.run([str(EXE), *args], cwd=cwd, capture_output=True, text=True, timeout=120)
+
+
+def test_cli_usage_and_errors(tmp_path):
+    assert EXE.exists(), "sim_tcpc is not built"
+    r = _run(["--help"], tmp_path)
+    assert r.returncode == 0 and "--ivc-mls" in r.stderr
+    r = _run(["--bogus"], tmp_path)
+    assert r.returncode == 1 and "Unknown argument '--bogus'" in r.stderr
+    r = _run(["--ivc-mls"], tmp_path)
+    assert r.returncode == 1 and "Missing value for --ivc-mls" in r.stderr
+    r = _run(["--geometry", "missing.vtk"], tmp_path)
+    assert r.returncode == 1 and "is not a legacy VTK RectilinearGrid file" in r.stderr
+    g = tcpc_mask(0.25)
+    write_wall_vtk(tmp_path / "geometry.vtk", g)
+    r = _run(["--material", "water"], tmp_path)
+    assert r.returncode == 1 and "Unknown material 'water'" in r.stderr
+
+
+@pytest.mark.parametrize("binary", [True, False])
+def test_cli_reads_geometry_and_logs_labels(tmp_path, binary):
+    """Up to the device the run is host-only: the log must carry the label
+    counts of the library's own setup (on a GPU host the run then finishes)."""
+    g = tcpc_mask(0.25)
+    write_wall_vtk(tmp_path / "geometry.vtk", g, binary=binary)
+    r = _run(["--case-id", "t1", "--steps", "2", "--vtk-period", "-1", "--print-period", "-1"], tmp_path)
+    assert r.returncode in (0, 1), r.stderr
+    log = (tmp_path / "results_t1" / "log_rank000").read_text()
+    _, info = lbm355.labels(g.flat(), g.dims, storage=1, nu_ref=1e-4)
+    assert (f"geometry labels: fluid={info['fluid']} wall={info['wall']} IVC(label3)={info['area_ivc']} "
+            f"SVC(label4)={info['area_svc']} outlet5={info['area_out5']} outlet6={info['area_out6']}") in log
+    assert "case=t1 material=newtonian density=1.000000e+03" in log
+    assert "prescribed flows: IVC=8.000000e+00 ml/s SVC=4.833333e+00 ml/s" in log
+    first = log.splitlines()[0]
+    assert first[4] == "|" and first[7] == "|" and first[13] == ":"    # YYYY|MM|DD HH:MM:SS.mmm
+
+
+def test_flows_known_velocity(tmp_path):
+    X, Y, Z = 20, 16, 12
+    dx = 1e-3
+    wall = np.ones((X, Y, Z))
+    vel = np.zeros((X, Y, Z, 3))
+    wall[0, 5:8, 4:6] = 3
+    vel[0, 5:8, 4:6, 0] = 0.2       # IVC, 6 cells flowing +x (inwards)
+    wall[X - 1, 9:11, 4:6] = 4
+    vel[X - 1, 9:11, 4:6, 0] = -0.1  # SVC, 4 cells flowing -x (inwards)
+    wall[8:10, 0, 4:6] = 5
+    vel[8:10, 0, 4:6, 1] = -0.14    # RPA, 4 cells flowing -y (outwards)
+    wall[8:10, Y - 1, 4:6] = 6
+    vel[8:10, Y - 1, 4:6, 1] = 0.16  # LPA, 4 cells flowing +y (outwards)
+    row = flows.compute_arrays(wall, vel, dx)
+    a = dx * dx * 6e4
+    assert row["IVC_l_min"] == pytest.approx(6 * 0.2 * a) and row["IVC_cells"] == 6
+    assert row["SVC_l_min"] == pytest.approx(4 * 0.1 * a)
+    assert row["RPA_l_min"] == pytest.approx(4 * 0.14 * a) and row["LPA_l_min"] == pytest.approx(4 * 0.16 * a)
+    assert row["mass_balance_l_min"] == pytest.approx((1.2 + 0.4 - 0.56 - 0.64) * a)
+    assert row["lpa_split_percent"] == pytest.approx(100 * 0.64 / 1.2)
+    assert row["rpa_split_percent"] + row["lpa_split_percent"] == pytest.approx(100.0)
+    assert flows.parse_label_map("3:IVC,7:OUT") == {3: "IVC", 7: "OUT"}

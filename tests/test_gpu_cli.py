@@ -1,0 +1,77 @@
+"""sim_tcpc end to end on the GPU: the executable's fields equal an in-process
+Case run on the same descriptor, its output tree and counters follow
+execute() (core.h:1476-1628), the flow post-processor recovers the prescribed
+inflows, and a zero-area geometry exits with code 2."""
+from __future__ import annotations
+
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from nonnewlbm_amd import _abi, flows
+from nonnewlbm_amd.build import EXE
+from nonnewlbm_amd.geometry import read_output_vtk, tcpc_mask, write_wall_vtk
+from nonnewlbm_amd.lbm355 import Case
+from nonnewlbm_amd.materials import lattice_units, material_by_name
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(args, cwd):
+    print("sim_tcpc", " ".join(args), flush=True)
+    return subprocess.run([str(EXE), *args], cwd=cwd, capture_output=True, text=True, timeout=300)
+
+
+def test_cli_matches_library_and_writes_outputs(gpu_lib, tmp_path):
+    g = tcpc_mask(0.25)
+    write_wall_vtk(tmp_path / "geometry.vtk", g)
+    mat, nu_lbm, ivc, svc = "sx", 0.01, 0.5, 0.3
+    u = lattice_units(material_by_name(mat), np.float32(g.dl).item(), nu_lbm)
+    steps = 60
+    # vtk every 20 steps -> files 0 (t=0), 1, 2, 3
+    args = ["--material", mat, "--case-id", "c1", "--ivc-mls", str(ivc), "--svc-mls", str(svc),
+            "--lbm-viscosity", str(nu_lbm), "--steps", str(steps), "--vtk-period", str(20 * u.dt * 0.999999),
+            "--print-period", str(30 * u.dt), "--precision", "fp32"]
+    r = _run(args, tmp_path)
+    assert r.returncode == 0, r.stderr + r.stdout
+    res = tmp_path / "results_c1"
+    log = (res / "log_rank000").read_text()
+    assert "START: simulation LBM:CUM dimensions 64 x 48 x 32" in log
+    assert "requested steps reached" in log and "MLUPS=" in log
+    files = sorted((res / "vtk3D").glob("rank000_data_*.vtk"))
+    assert [f.name for f in files] == [f"rank000_data_{i}.vtk" for i in range(4)]
+    out = read_output_vtk(res / "vtk3D" / "rank000_data_3.vtk")
+    assert out["field"]["CYCLE"][0] == 3
+    # the same case through the library
+    dl = float(np.float32(g.dl))
+    dt = u.dt
+    kw = dict(dl=dl, dt=dt, collision=_abi.CUM, storage=_abi.F32, nu_ref=nu_lbm, cy_enabled=1, cy_model=_abi.CY_NEQ,
+              cy_nu0=u.nu0, cy_lambda=u.lam, cy_a=u.a, cy_n=u.n, profile=_abi.PROFILE_PLUG,
+              ivc_flux=ivc * 1e-6 / (dl * dl * dl) * dt, svc_flux=svc * 1e-6 / (dl * dl * dl) * dt)
+    c = Case(g.flat(), g.dims, **kw)
+    c.step(steps)
+    lab = c.field("label")
+    vconv = dl / dt
+    for k, name in enumerate(("vx", "vy", "vz")):
+        ref = (c.field(name) * vconv).astype(np.float32)
+        np.testing.assert_array_equal(out["velocity"][..., k], ref, err_msg=name)
+    np.testing.assert_array_equal(out["wall"], lab.astype(np.float32))
+    # compute_flows on the output: plug inflows are exact up to float32 output
+    row = flows.compute_file(res / "vtk3D" / "rank000_data_3.vtk")
+    assert row["IVC_l_min"] == pytest.approx(ivc * 0.06, rel=1e-5)
+    assert row["SVC_l_min"] == pytest.approx(svc * 0.06, rel=1e-5)
+    assert row["IVC_cells"] == c.info()["area_ivc"]
+
+
+def test_cli_zero_area_terminates(gpu_lib, tmp_path):
+    g = tcpc_mask(0.25)
+    g.labels[:] = 1
+    write_wall_vtk(tmp_path / "geometry.vtk", g)
+    r = _run(["--case-id", "z", "--steps", "10", "--vtk-period", "1e9"], tmp_path)
+    assert r.returncode == 2, r.stderr + r.stdout
+    log = (tmp_path / "results_z" / "log_rank000").read_text()
+    assert "error: zero inlet area detected: left=0 right=0" in log
+    assert "terminate flag triggered" in log
+    assert (tmp_path / "results_z" / "vtk3D" / "rank000_data_0.vtk").exists()
