@@ -65,6 +65,98 @@ def make_case(scale: float, storage: int, collision: int, lbm_viscosity: float):
     return g, kw
 
 
+def sweep_requests(n: int, rate: float, seed: int = 0) -> list[dict]:
+    """Synthetic serving sweep: TCPC junctions at scales 0.5 / 0.75 / 1.0,
+    every collision model, Newtonian / CY-A / CY-B, FP32 or FP16-offset
+    storage, parabolic inflows u_max ~ U[0.01, 0.06], 1000-6000 steps,
+    Poisson arrivals at `rate` per second (numpy default_rng(seed))."""
+    from nonnewlbm_amd import _abi
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for k in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        out.append(dict(arrival=t, scale=float(rng.choice([0.5, 0.75, 1.0])),
+                        collision=int(rng.choice([_abi.BGK, _abi.MRT, _abi.KBC, _abi.CUM])),
+                        material=str(rng.choice(["blood-newtonian", "cy-a", "cy-b"])),
+                        storage=int(rng.choice([_abi.F32, _abi.F16_OFFSET])),
+                        steps=int(rng.integers(1000, 6001)),
+                        umax=(float(rng.uniform(0.01, 0.06)), float(rng.uniform(0.01, 0.06)))))
+    return out
+
+
+def _sweep_case(req: dict, geoms: dict, storage=None):
+    from nonnewlbm_amd import _abi
+    from nonnewlbm_amd.geometry import tcpc_mask
+    from nonnewlbm_amd.materials import case_kwargs, material_by_name
+    sc = req["scale"]
+    if sc not in geoms:
+        geoms[sc] = tcpc_mask(sc)
+    g = geoms[sc]
+    kw = case_kwargs(material_by_name(req["material"]), g.dl, 0.01)
+    kw.update(storage=req["storage"] if storage is None else storage, collision=req["collision"],
+              profile=_abi.PROFILE_PARABOLIC, ivc_umax=req["umax"][0], svc_umax=req["umax"][1],
+              ivc_center=g.meta["ivc"]["center"], svc_center=g.meta["svc"]["center"],
+              ivc_radius=g.meta["ivc"]["radius"], svc_radius=g.meta["svc"]["radius"], stats_every=1)
+    return g, kw
+
+
+def sweep(n: int, rate: float, tick: int) -> dict:
+    """Requests arrive on their Poisson schedule into one Pool; latency =
+    completion - submission on the pool clock.  Baseline: the same requests
+    run back to back, one plain FP32 case at a time."""
+    from nonnewlbm_amd.lbm355 import Case, Pool
+    reqs = sweep_requests(n, rate)
+    geoms: dict = {}
+    specs = [_sweep_case(r, geoms) for r in reqs]
+    pool = Pool(steps_per_tick=tick, graph=True)
+    t0 = time.perf_counter()
+    ids = []
+    for r, (g, kw) in zip(reqs, specs):
+        wait = r["arrival"] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        ids.append(pool.submit(g.flat(), g.dims, r["steps"], report_every=1000, **kw))
+    pool.drain()
+    sts = [pool.poll(i) for i in ids]
+    info = pool.stats()
+    pool.close()
+    done = [s for s in sts if s["state_name"] == "done"]
+    lat = np.array([s["t_done"] - s["t_submit"] for s in done])
+    first = np.array([s["t_first_report"] - s["t_submit"] for s in done])
+    span = max(s["t_done"] for s in done) - min(s["t_submit"] for s in done)
+    updates = info["cell_updates"]
+    print(f"# sweep: {len(done)}/{n} done, p50 {np.percentile(lat, 50):.3f} s, p99 {np.percentile(lat, 99):.3f} s, "
+          f"{updates / info['busy_seconds'] / 1e6:.0f} fluid MLUPS while stepping", file=sys.stderr, flush=True)
+    # sequential plain FP32 baseline over the same requests
+    from nonnewlbm_amd import _abi
+    t1 = time.perf_counter()
+    seq_updates = 0
+    for r in reqs:
+        g, kw = _sweep_case(r, geoms, storage=_abi.F32)
+        c = Case(g.flat(), g.dims, **kw)
+        c.step(r["steps"])
+        c.sync()
+        seq_updates += c.info()["active_cells"] * r["steps"]
+        c.close()
+    seq_s = time.perf_counter() - t1
+    return {
+        "requests": n, "done": len(done), "failed": n - len(done), "arrival_rate_per_s": rate,
+        "steps_per_tick": tick, "seed": 0,
+        "workload": "TCPC junctions at scale 0.5/0.75/1.0 (128x96x64 .. 256x192x128), BGK/MRT/KBC/CUM, "
+                    "Newtonian/CY-A/CY-B, FP32/FP16-offset, parabolic u_max U[0.01,0.06], 1000-6000 steps",
+        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p95": round(float(np.percentile(lat, 95)), 4),
+                      "p99": round(float(np.percentile(lat, 99)), 4), "max": round(float(lat.max()), 4)},
+        "first_report_s": {"p50": round(float(np.percentile(first, 50)), 4),
+                           "p99": round(float(np.percentile(first, 99)), 4)},
+        "makespan_s": round(span, 3),
+        "fluid_mlups_busy": round(updates / info["busy_seconds"] / 1e6, 1),
+        "fluid_mlups_makespan": round(updates / span / 1e6, 1),
+        "sequential_fp32_fluid_mlups": round(seq_updates / seq_s / 1e6, 1),
+        "sequential_fp32_seconds": round(seq_s, 3),
+    }
+
+
 def cpu_baseline(g, kw, budget_s: float) -> dict:
     from oracle.oracle import OracleCase
 
@@ -94,6 +186,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-hbm", action="store_true", help="skip the 512x384x256 HBM-resident line")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--sweep-requests", type=int, default=48)
+    ap.add_argument("--sweep-rate", type=float, default=12.0, help="Poisson arrivals per second")
+    ap.add_argument("--sweep-tick", type=int, default=64, help="steps per pool tick")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -207,6 +303,9 @@ def main() -> int:
         print(f"# HBM-resident: k_step {s2 * 1e3:.1f} us -> {a2:.0f} GB/s ({a2 / PEAK_GBS:.1%})", file=sys.stderr,
               flush=True)
         c2.close()
+
+    if not args.no_sweep:
+        out["sweep"] = sweep(args.sweep_requests, args.sweep_rate, args.sweep_tick)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, kw, args.cpu_seconds)

@@ -186,6 +186,10 @@ int lbm355_case_set_inflow(lbm355_case* c, double ivc_flux, double svc_flux);
    case's stream; ends with the statistics and macro fields of the last step */
 int lbm355_case_step(lbm355_case* c, int64_t n);
 int lbm355_case_sync(lbm355_case* c);
+/* launch shape of the case's fused step: 4x4x4 bricks (wavefronts) per
+   workgroup, 1, 2, 4 or 8 (default 4).  No counterpart in the reference, whose
+   launch shape is the y-block size (--block-size, sim_tcpc.cu:36). */
+int lbm355_case_set_shape(lbm355_case* c, int bricks_per_group);
 
 /* dense double copy of one field; bytes must be >= nx*ny*nz*8 */
 int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes);
@@ -229,9 +233,67 @@ typedef struct lbm355_batch lbm355_batch;
 int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batch** out);
 int lbm355_batch_step(lbm355_batch* b, int64_t n);
 int lbm355_batch_destroy(lbm355_batch* b);
-/* launch shape: bricks per workgroup (1,2,4,8); 0 = autotuned */
+/* launch shape: 4x4x4 bricks (wavefronts) per workgroup, 1, 2, 4 or 8 */
 int lbm355_batch_set_shape(lbm355_batch* b, int bricks_per_group);
+/* times every launch shape on 10 real steps of the batch each (2 warm-up + 8
+   timed; the steps count as simulation steps of every member) and keeps the
+   fastest */
 int lbm355_batch_autotune(lbm355_batch* b, int* best_bricks_per_group);
+
+/* --- serving pool ----------------------------------------------------------
+ * No counterpart in the reference, which runs one case per process and queues
+ * cases through Slurm (scripts/slurm_run_cases.py:104-198).  A pool owns one
+ * worker thread that drives the current device: requests wait in FIFO order
+ * until their footprint fits the device-memory budget; admitted cases of one
+ * collision and storage type are stepped by one fused launch per step over
+ * their concatenated brick lists, all types within the same tick; finished
+ * cases leave between ticks and free their memory for waiting requests
+ * (continuous batching).  Pool calls are thread-safe. */
+typedef struct lbm355_pool lbm355_pool;
+enum {
+    LBM355_REQ_QUEUED = 0,
+    LBM355_REQ_RUNNING = 1,
+    LBM355_REQ_DONE = 2,
+    LBM355_REQ_FAILED = 3
+};
+typedef struct lbm355_request_status {
+    int32_t state;              /* LBM355_REQ_*                                      */
+    int32_t error;              /* status code of a failed request (ETERMINATE for a
+                                   zero-area case, ENOMEM over budget, ...)           */
+    int64_t steps_done, steps;
+    int64_t active_cells, device_bytes;
+    double t_submit, t_start;   /* seconds on the pool clock (0 = pool creation)     */
+    double t_first_report;      /* first plane-flux report (every report_every steps) */
+    double t_done;
+    double flux[4];             /* last reported fluxes, as lbm355_case_flux(mean=0)  */
+    double mean_flux[4];        /* at completion, lbm355_case_flux(mean=1)            */
+    char message[128];          /* why a request failed                               */
+} lbm355_request_status;
+typedef struct lbm355_pool_info {
+    int64_t submitted, queued, running, done, failed;
+    int64_t budget_bytes, used_bytes;
+    int64_t cell_updates;       /* sum of active cells x steps over all requests      */
+    int64_t ticks;
+    double busy_seconds;        /* time spent stepping                               */
+    double clock;               /* pool clock now                                    */
+} lbm355_pool_info;
+/* budget_bytes <= 0: 90 % of the free device memory; steps_per_tick <= 0: 64;
+   graph = 1 replays captured hipGraphs inside a batch */
+int lbm355_pool_create(int64_t budget_bytes, int steps_per_tick, int graph, lbm355_pool** out);
+/* copies the descriptor and geometry; report_every > 0 reports plane fluxes
+   every so many steps; keep = 1 keeps the finished case for lbm355_pool_take */
+int lbm355_pool_submit(lbm355_pool* p, const lbm355_case_desc* desc, int64_t steps, int64_t report_every,
+                       int keep, int64_t* id);
+/* LBM355_OK when the request is done or failed, LBM355_EBUSY while queued or running */
+int lbm355_pool_poll(lbm355_pool* p, int64_t id, lbm355_request_status* st);
+/* timeout_s < 0 waits without limit */
+int lbm355_pool_wait(lbm355_pool* p, int64_t id, double timeout_s, lbm355_request_status* st);
+int lbm355_pool_drain(lbm355_pool* p, double timeout_s);
+/* hands a finished keep=1 case to the caller, who destroys it */
+int lbm355_pool_take(lbm355_pool* p, int64_t id, lbm355_case** out);
+int lbm355_pool_stats(lbm355_pool* p, lbm355_pool_info* st);
+/* stops the worker after its current tick; unfinished requests are dropped */
+int lbm355_pool_destroy(lbm355_pool* p);
 
 /* bytes of device memory a descriptor would need (admission control) */
 int lbm355_case_footprint(const lbm355_case_desc* desc, int64_t* bytes);

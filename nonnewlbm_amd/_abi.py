@@ -45,7 +45,7 @@ FIELD = {
 
 OK = 0
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "ETERMINATE", -5: "ENOTFOUND", -6: "EBUSY"}
-ETERMINATE = -4
+EINVAL, ENOMEM, EDEVICE, ETERMINATE, ENOTFOUND, EBUSY = -1, -2, -3, -4, -5, -6
 
 
 class CaseDesc(C.Structure):
@@ -89,6 +89,40 @@ class OutputDesc(C.Structure):
         ("phys_density", C.c_double), ("phys_nu_inf", C.c_double), ("time", C.c_double),
         ("cycle", C.c_int32), ("stat_steps", C.c_int32), ("reserved", C.c_int32 * 6),
     ]
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+REQ_STATES = {REQ_QUEUED: "queued", REQ_RUNNING: "running", REQ_DONE: "done", REQ_FAILED: "failed"}
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32), ("error", C.c_int32),
+        ("steps_done", C.c_int64), ("steps", C.c_int64),
+        ("active_cells", C.c_int64), ("device_bytes", C.c_int64),
+        ("t_submit", C.c_double), ("t_start", C.c_double), ("t_first_report", C.c_double), ("t_done", C.c_double),
+        ("flux", C.c_double * 4), ("mean_flux", C.c_double * 4),
+        ("message", C.c_char * 128),
+    ]
+
+    def as_dict(self) -> dict:
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["flux"] = list(self.flux)
+        d["mean_flux"] = list(self.mean_flux)
+        d["message"] = self.message.decode(errors="replace")
+        d["state_name"] = REQ_STATES.get(self.state, "?")
+        return d
+
+
+class PoolInfo(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64), ("queued", C.c_int64), ("running", C.c_int64), ("done", C.c_int64),
+        ("failed", C.c_int64), ("budget_bytes", C.c_int64), ("used_bytes", C.c_int64),
+        ("cell_updates", C.c_int64), ("ticks", C.c_int64), ("busy_seconds", C.c_double), ("clock", C.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 def make_desc(geometry_flat: np.ndarray, dims, **kw) -> tuple[CaseDesc, np.ndarray]:
@@ -155,6 +189,7 @@ def load() -> C.CDLL:
         "lbm355_case_set_inflow": (C.c_int, [P, C.c_double, C.c_double]),
         "lbm355_case_step": (C.c_int, [P, C.c_int64]),
         "lbm355_case_sync": (C.c_int, [P]),
+        "lbm355_case_set_shape": (C.c_int, [P, C.c_int]),
         "lbm355_case_get_field": (C.c_int, [P, C.c_int, C.POINTER(C.c_double), C.c_size_t]),
         "lbm355_case_get_populations": (C.c_int, [P, C.POINTER(C.c_double), C.c_size_t]),
         "lbm355_case_compute_strain": (C.c_int, [P]),
@@ -170,6 +205,15 @@ def load() -> C.CDLL:
         "lbm355_case_labels": (C.c_int, [C.POINTER(CaseDesc), C.POINTER(C.c_int32), C.POINTER(CaseInfo)]),
         "lbm355_case_time_kernel": (C.c_int, [P, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "lbm355_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+        "lbm355_pool_create": (C.c_int, [C.c_int64, C.c_int, C.c_int, C.POINTER(P)]),
+        "lbm355_pool_submit": (C.c_int, [P, C.POINTER(CaseDesc), C.c_int64, C.c_int64, C.c_int,
+                                         C.POINTER(C.c_int64)]),
+        "lbm355_pool_poll": (C.c_int, [P, C.c_int64, C.POINTER(RequestStatus)]),
+        "lbm355_pool_wait": (C.c_int, [P, C.c_int64, C.c_double, C.POINTER(RequestStatus)]),
+        "lbm355_pool_drain": (C.c_int, [P, C.c_double]),
+        "lbm355_pool_take": (C.c_int, [P, C.c_int64, C.POINTER(P)]),
+        "lbm355_pool_stats": (C.c_int, [P, C.POINTER(PoolInfo)]),
+        "lbm355_pool_destroy": (C.c_int, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -183,11 +227,13 @@ def load() -> C.CDLL:
 EXPORTED = [
     "lbm355_version", "lbm355_last_error", "lbm355_device_count",
     "lbm355_case_create", "lbm355_case_destroy", "lbm355_case_info_get", "lbm355_case_set_inflow",
-    "lbm355_case_step", "lbm355_case_sync", "lbm355_case_get_field", "lbm355_case_get_populations",
+    "lbm355_case_step", "lbm355_case_sync", "lbm355_case_set_shape", "lbm355_case_get_field", "lbm355_case_get_populations",
     "lbm355_case_compute_strain", "lbm355_case_flux", "lbm355_case_reset_stats", "lbm355_case_write_vtk",
     "lbm355_batch_create", "lbm355_batch_step", "lbm355_batch_destroy", "lbm355_batch_set_shape",
     "lbm355_batch_autotune", "lbm355_case_footprint", "lbm355_case_labels", "lbm355_case_time_kernel",
     "lbm355_copy_bandwidth",
+    "lbm355_pool_create", "lbm355_pool_submit", "lbm355_pool_poll", "lbm355_pool_wait", "lbm355_pool_drain",
+    "lbm355_pool_take", "lbm355_pool_stats", "lbm355_pool_destroy",
 ]
 
 

@@ -40,7 +40,7 @@ def _stale(target: Path, sources: list[Path]) -> bool:
 
 
 def build_library(force: bool = True) -> Path:
-    srcs = [CSRC / "lbm355_kernels.hip", CSRC / "lbm355_host.cpp"]
+    srcs = [CSRC / "lbm355_kernels.hip", CSRC / "lbm355_host.cpp", CSRC / "lbm355_pool.cpp"]
     deps = srcs + [CSRC / "lbm355_internal.h", ROOT / "include" / "lbm355.h"]
     if not force and not _stale(LIB, deps):
         return LIB

@@ -798,6 +798,13 @@ int lbm355_case_step(lbm355_case* c, int64_t n)
     return LBM355_OK;
 }
 
+int lbm355_case_set_shape(lbm355_case* c, int bpg)
+{
+    if (!c || !(bpg == 1 || bpg == 2 || bpg == 4 || bpg == 8)) return fail(LBM355_EINVAL, "bricks per group");
+    c->bricks_per_group = bpg;
+    return LBM355_OK;
+}
+
 int lbm355_case_sync(lbm355_case* c)
 {
     if (!c) return fail(LBM355_EINVAL, "null case");
@@ -1268,7 +1275,14 @@ int lbm355_batch_set_shape(lbm355_batch* b, int bpg)
     return LBM355_OK;
 }
 
-int lbm355_batch_step(lbm355_batch* b, int64_t n)
+}  // extern "C"
+
+// the library's error channel for the other host sources
+int lbm355_set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+// queue n steps of a batch on its stream without waiting (the pool enqueues
+// every batch of a tick, then finishes them)
+int batch_enqueue(lbm355_batch* b, int64_t n)
 {
     if (!b) return fail(LBM355_EINVAL, "null batch");
     if (n <= 0) return LBM355_OK;
@@ -1332,19 +1346,62 @@ int lbm355_batch_step(lbm355_batch* b, int64_t n)
             c->stats_steps += (c->steps + n) / k - c->steps / k;
         }
         c->steps += n;
-        c->macro_valid_step = -2;  // rho,u live in the batch's macro request; see batch_step
+        c->macro_valid_step = -2;  // rho,u of the last step land once the stream is done
         c->strain_valid = false;
     }
+    return LBM355_OK;
+}
+
+// wait for a batch's queued steps; the macro fields of every member then hold
+// rho,u of its last step
+int batch_finish(lbm355_batch* b)
+{
     HIP_TRY(hipStreamSynchronize(b->stream));
     for (lbm355_case* c : b->cases) c->macro_valid_step = c->steps - 1;
     return LBM355_OK;
 }
 
+extern "C" {
+
+int lbm355_batch_step(lbm355_batch* b, int64_t n)
+{
+    int rc = batch_enqueue(b, n);
+    if (rc) return rc;
+    return n > 0 ? batch_finish(b) : LBM355_OK;
+}
+
 int lbm355_batch_autotune(lbm355_batch* b, int* best)
 {
     if (!b) return fail(LBM355_EINVAL, "null batch");
-    (void)best;
-    return fail(LBM355_EINVAL, "autotune not available in this build");
+    // real steps of the batch, an even number per shape so both AA parities
+    // count; the fastest shape is kept
+    const int shapes[4] = {1, 2, 4, 8};
+    const int steps = 8;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    float best_ms = 0.f;
+    int best_bpg = b->bricks_per_group;
+    int rc = LBM355_OK;
+    for (int k = 0; k < 4 && rc == LBM355_OK; ++k) {
+        b->bricks_per_group = shapes[k];
+        drop_graph(*b);
+        if ((rc = batch_enqueue(b, 2)) || (rc = batch_finish(b))) break;  // warm-up (and graph capture)
+        if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
+        if ((rc = batch_enqueue(b, steps))) break;
+        if (hipEventRecord(e1, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
+        if ((rc = batch_finish(b))) break;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (k == 0 || ms < best_ms) { best_ms = ms; best_bpg = shapes[k]; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    b->bricks_per_group = best_bpg;
+    drop_graph(*b);
+    if (best) *best = best_bpg;
+    return LBM355_OK;
 }
 
 int lbm355_batch_destroy(lbm355_batch* b)

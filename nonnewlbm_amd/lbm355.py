@@ -37,6 +37,19 @@ class Case:
         self.h = h
         self._geom = None  # the library copied what it needs
 
+    @classmethod
+    def from_handle(cls, handle: C.c_void_p, dims: Sequence[int]) -> "Case":
+        """Wrap a case created elsewhere (a finished pool request); the Case
+        owns it from now on."""
+        self = cls.__new__(cls)
+        self.lib = load()
+        self.dims = tuple(int(v) for v in dims)
+        self.desc = None
+        self._geom = None
+        self.terminate = False
+        self.h = handle
+        return self
+
     # -- lifetime
     def close(self) -> None:
         if getattr(self, "h", None):
@@ -64,6 +77,10 @@ class Case:
 
     def sync(self) -> None:
         check(self.lib.lbm355_case_sync(self.h), "sync")
+
+    def set_shape(self, bricks_per_group: int) -> None:
+        """4x4x4 bricks per workgroup of the fused step (1, 2, 4 or 8)."""
+        check(self.lib.lbm355_case_set_shape(self.h, int(bricks_per_group)), "case_set_shape")
 
     def time_kernel(self, n: int) -> tuple[float, float]:
         """Run n steps; returns (ms per whole step, ms per fused-kernel launch)."""
@@ -139,6 +156,70 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "h", None):
             self.lib.lbm355_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Pool:
+    """Serving pool (lbm355_pool_*): requests are admitted against a device
+    memory budget and stepped together, one fused launch per step and type."""
+
+    def __init__(self, budget_bytes: int = 0, steps_per_tick: int = 64, graph: bool = True):
+        self.lib = load()
+        h = C.c_void_p()
+        check(self.lib.lbm355_pool_create(int(budget_bytes), int(steps_per_tick), 1 if graph else 0, C.byref(h)),
+              "pool_create")
+        self.h = h
+        self._dims: dict[int, tuple] = {}
+
+    def submit(self, geometry_flat: np.ndarray, dims: Sequence[int], steps: int, report_every: int = 0,
+               keep: bool = False, **kw) -> int:
+        d, keep_alive = make_desc(geometry_flat, dims, **kw)
+        rid = C.c_int64()
+        check(self.lib.lbm355_pool_submit(self.h, C.byref(d), int(steps), int(report_every), 1 if keep else 0,
+                                          C.byref(rid)), "pool_submit")
+        del keep_alive  # the pool copied the geometry
+        self._dims[rid.value] = tuple(int(v) for v in dims)
+        return rid.value
+
+    def poll(self, rid: int) -> dict:
+        st = _abi.RequestStatus()
+        rc = self.lib.lbm355_pool_poll(self.h, int(rid), C.byref(st))
+        if rc not in (_abi.OK, _abi.EBUSY):
+            check(rc, "pool_poll")
+        return st.as_dict()
+
+    def wait(self, rid: int, timeout: float = -1.0) -> dict:
+        st = _abi.RequestStatus()
+        rc = self.lib.lbm355_pool_wait(self.h, int(rid), float(timeout), C.byref(st))
+        if rc not in (_abi.OK, _abi.EBUSY):
+            check(rc, "pool_wait")
+        return st.as_dict()
+
+    def drain(self, timeout: float = -1.0) -> bool:
+        rc = self.lib.lbm355_pool_drain(self.h, float(timeout))
+        if rc not in (_abi.OK, _abi.EBUSY):
+            check(rc, "pool_drain")
+        return rc == _abi.OK
+
+    def take(self, rid: int) -> Case:
+        h = C.c_void_p()
+        check(self.lib.lbm355_pool_take(self.h, int(rid), C.byref(h)), "pool_take")
+        return Case.from_handle(h, self._dims[int(rid)])
+
+    def stats(self) -> dict:
+        st = _abi.PoolInfo()
+        check(self.lib.lbm355_pool_stats(self.h, C.byref(st)), "pool_stats")
+        return st.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.lbm355_pool_destroy(self.h)
             self.h = None
 
     def __del__(self):

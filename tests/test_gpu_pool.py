@@ -1,0 +1,86 @@
+"""Serving pool on the GPU: every request's result equals the same case run
+alone (continuous batching changes nothing in the arithmetic), admission
+respects the memory budget, and failures stay per request."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from nonnewlbm_amd import _abi
+from nonnewlbm_amd.lbm355 import Case, Pool, footprint
+from nonnewlbm_amd.materials import case_kwargs, material_by_name
+from tests.helpers import tcpc_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs():
+    out = []
+    for scale, storage, collision, steps, cy in ((0.25, _abi.F32, _abi.CUM, 70, None),
+                                                 (0.3, _abi.F16_OFFSET, _abi.KBC, 33, None),
+                                                 (0.25, _abi.F32, _abi.CUM, 41, "cy-a"),
+                                                 (0.3, _abi.F32, _abi.BGK, 57, None),
+                                                 (0.25, _abi.BF16_OFFSET, _abi.MRT, 20, None)):
+        g, kw = tcpc_case(scale, storage=storage, collision=collision)
+        if cy:
+            kw.update(case_kwargs(material_by_name(cy), g.dl, 0.01))
+        out.append((g, kw, steps))
+    return out
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_pool_results_equal_alone(gpu_lib, graph):
+    specs = _specs()
+    pool = Pool(steps_per_tick=16, graph=graph)
+    ids = [pool.submit(g.flat(), g.dims, steps, report_every=10, keep=True, **kw) for g, kw, steps in specs]
+    assert pool.drain(timeout=120.0)
+    for rid, (g, kw, steps) in zip(ids, specs):
+        st = pool.poll(rid)
+        assert st["state"] == _abi.REQ_DONE, st
+        assert st["steps_done"] == steps
+        assert 0 < st["t_first_report"] <= st["t_done"]
+        got = pool.take(rid)
+        alone = Case(g.flat(), g.dims, **kw)
+        alone.step(steps)
+        for name in ("rho", "vx", "vy", "vz", "sum_rux"):
+            np.testing.assert_array_equal(got.field(name), alone.field(name), err_msg=f"{rid} {name}")
+        np.testing.assert_allclose(st["flux"], alone.flux(mean=False), rtol=0, atol=0)
+        got.close()
+        alone.close()
+    info = pool.stats()
+    assert info["done"] == len(specs) and info["failed"] == 0 and info["used_bytes"] == 0
+    assert info["cell_updates"] == sum(pool.poll(r)["active_cells"] * s for r, (_, _, s) in zip(ids, specs))
+    pool.close()
+
+
+def test_pool_admission_budget_and_failures(gpu_lib):
+    g, kw = tcpc_case(0.25, storage=_abi.F32)
+    fp = footprint(g.flat(), g.dims, **kw)
+    budget = int(2.5 * fp)                       # at most two of these at a time
+    pool = Pool(budget_bytes=budget, steps_per_tick=8)
+    ids = [pool.submit(g.flat(), g.dims, 24, **dict(kw, ivc_flux=kw["ivc_flux"] * (1 + 0.1 * k))) for k in range(5)]
+    big, kbig = tcpc_case(0.5, storage=_abi.F64)
+    too_big = pool.submit(big.flat(), big.dims, 10, **kbig)     # larger than the whole budget
+    wall = g.labels.copy()
+    wall[:] = 1
+    zero = pool.submit(wall.transpose(2, 1, 0).ravel(), g.dims, 10, **kw)  # zero inlet area
+    assert pool.drain(timeout=120.0)
+    sts = [pool.poll(r) for r in ids]
+    assert all(s["state"] == _abi.REQ_DONE for s in sts)
+    # FIFO admission under the budget: never more than two cases resident
+    events = sorted([(s["t_start"], 1) for s in sts] + [(s["t_done"], -1) for s in sts], key=lambda e: (e[0], e[1]))
+    live = peak = 0
+    for _, d in events:
+        live += d
+        peak = max(peak, live)
+    assert peak <= 2
+    assert sts[2]["t_start"] >= min(sts[0]["t_done"], sts[1]["t_done"])
+    b = pool.poll(too_big)
+    assert b["state"] == _abi.REQ_FAILED and b["error"] == _abi.ENOMEM and "budget" in b["message"]
+    z = pool.poll(zero)
+    assert z["state"] == _abi.REQ_FAILED and z["error"] == _abi.ETERMINATE
+    info = pool.stats()
+    assert info["done"] == 5 and info["failed"] == 2 and info["used_bytes"] == 0 and info["budget_bytes"] == budget
+    with pytest.raises(_abi.LbmError):
+        pool.take(ids[0])                        # not submitted with keep
+    pool.close()
