@@ -157,6 +157,60 @@ def sweep(n: int, rate: float, tick: int) -> dict:
     }
 
 
+def precision_study(scale: float, steps: int) -> dict:
+    """Storage precision against FP64 on the pulsatile junction: cumulant +
+    CY-A (neq), parabolic inflows 0.04/0.03 with u(t) = u (1 + 0.3 sin(2 pi t /
+    4000)), FP64 statistics.  Relative L2 over the non-wall cells of u and of
+    the output-path gamma_dot and wss_proxy (the VTK formulas, sim_tcpc.cu:
+    697-714), and the LPA share of the mean outlet flow."""
+    from nonnewlbm_amd import _abi
+    from nonnewlbm_amd.lbm355 import Case
+    from nonnewlbm_amd.materials import material_by_name
+    mat = material_by_name("cy-a")
+    res: dict = {"workload": f"TCPC junction x{scale} pulsatile (A=0.3, T=4000 steps), CUM + CY-A neq, {steps} steps",
+                 "storages": {}}
+    ref = None
+    for st in (_abi.F64, _abi.F32, _abi.F16_OFFSET, _abi.BF16_OFFSET):
+        g, kw = make_case(scale, st, _abi.CUM, 0.01)
+        kw.update(pulse_amp=0.3, pulse_period=4000.0)
+        c = Case(g.flat(), g.dims, **kw)
+        t0 = time.perf_counter()
+        c.step(steps)
+        c.sync()
+        secs = time.perf_counter() - t0
+        lab = c.field("label")
+        act = lab != 1
+        u = np.stack([c.field(k)[act] for k in ("vx", "vy", "vz")])
+        c.compute_strain()
+        S = {k: c.field(k)[act] for k in ("s11", "s12", "s13", "s22", "s32", "s33")}
+        gam = np.sqrt(2.0) * np.sqrt(S["s11"] ** 2 + S["s22"] ** 2 + S["s33"] ** 2
+                                     + 2 * (S["s12"] ** 2 + S["s13"] ** 2 + S["s32"] ** 2))
+        dt, dl = kw["dt"], kw["dl"]
+        nu = kw["nu_ref"] + (kw["cy_nu0"] - kw["nu_ref"]) * (1 + (gam * kw["cy_lambda"]) ** kw["cy_a"]) ** (
+            (kw["cy_n"] - 1) / kw["cy_a"])
+        wallm = c.field("wallmask") != 0
+        p = np.pad(wallm, 1, constant_values=False)
+        near = (p[:-2, 1:-1, 1:-1] | p[2:, 1:-1, 1:-1] | p[1:-1, :-2, 1:-1] | p[1:-1, 2:, 1:-1]
+                | p[1:-1, 1:-1, :-2] | p[1:-1, 1:-1, 2:])[act]
+        gdot = gam / dt
+        wss = np.where(near, nu * dl * dl / dt * mat.density * gdot, 0.0)
+        mf = c.flux(mean=True)
+        split = mf[3] / (mf[2] + mf[3])
+        c.close()
+        row = {"seconds": round(secs, 3), "fluid_mlups": round(int(act.sum()) * steps / secs / 1e6, 1),
+               "lpa_split": round(float(split), 6)}
+        if ref is None:
+            ref = (u, gdot, wss, split)
+        else:
+            def rl2(a, b):
+                return float(np.sqrt(np.sum((a - b) ** 2)) / np.sqrt(np.sum(b ** 2)))
+            row.update(u_rel_l2=rl2(u, ref[0]), gamma_dot_rel_l2=rl2(gdot, ref[1]), wss_rel_l2=rl2(wss, ref[2]),
+                       split_abs_diff_percent=round(100 * abs(split - ref[3]), 5))
+        res["storages"][_abi.STORAGE_NAMES[st]] = row
+        print(f"# precision {_abi.STORAGE_NAMES[st]}: {row}", file=sys.stderr, flush=True)
+    return res
+
+
 def cpu_baseline(g, kw, budget_s: float) -> dict:
     from oracle.oracle import OracleCase
 
@@ -187,6 +241,9 @@ def main() -> int:
     ap.add_argument("--no-hbm", action="store_true", help="skip the 512x384x256 HBM-resident line")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-sweep", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
+                    help="also run the FP64/FP32/FP16/BF16 storage study for STEPS steps")
+    ap.add_argument("--precision-scale", type=float, default=2.0)
     ap.add_argument("--sweep-requests", type=int, default=48)
     ap.add_argument("--sweep-rate", type=float, default=12.0, help="Poisson arrivals per second")
     ap.add_argument("--sweep-tick", type=int, default=64, help="steps per pool tick")
@@ -306,6 +363,9 @@ def main() -> int:
 
     if not args.no_sweep:
         out["sweep"] = sweep(args.sweep_requests, args.sweep_rate, args.sweep_tick)
+
+    if args.precision_study > 0:
+        out["precision_study"] = precision_study(args.precision_scale, args.precision_study)
 
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, kw, args.cpu_seconds)

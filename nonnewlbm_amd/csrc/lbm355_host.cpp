@@ -280,8 +280,6 @@ int validate(const lbm355_case_desc* d)
     if (d->collision < 0 || d->collision > 3) return fail(LBM355_EINVAL, "unknown collision");
     if (!(d->nu_ref > 0.0)) return fail(LBM355_EINVAL, "LBM viscosity must be positive (core.h:1044)");
     if (d->cy_enabled && (d->cy_model < 0 || d->cy_model > 2)) return fail(LBM355_EINVAL, "unknown cy_model");
-    if (d->cy_enabled && d->cy_model == LBM355_CY_FD_FORCE)
-        return fail(LBM355_EINVAL, "cy_model fd-force is not available in this build");
     return LBM355_OK;
 }
 
@@ -294,6 +292,7 @@ int64_t footprint_bytes(const lbm355_case_desc& d, int64_t n_active, int64_t n_s
     b += n_active * (kQ * 4 + 4 + 6);                     // nbr, bc_base, origin
     b += n_active * kMacroFields * kLanes * cb;           // macro
     b += n_active * 6 * kLanes * cb;                      // strain
+    if (d.cy_enabled && d.cy_model == LBM355_CY_FD_FORCE) b += n_active * 3 * kLanes * cb;  // body force
     if (d.stats_every > 0) b += n_active * kStatFields * kLanes * 8;
     b += n_bc * (4 * cb + 2 * kQ * 4 + 4 + 4 + 1 + 8);   // bc tables
     b += n_active * 8;                                    // brick list
@@ -507,6 +506,8 @@ int setup_device(lbm355_case& c)
     // rho = 1, u = 0 for the initial macro state
     if ((rc = alloc_zero(c, c.macro, size_t(std::max(na, 1)) * kMacroFields * kLanes * cb))) return rc;
     if ((rc = alloc_zero(c, c.strain, size_t(std::max(na, 1)) * 6 * kLanes * cb))) return rc;
+    const bool fd_force = d.cy_enabled && d.cy_model == LBM355_CY_FD_FORCE;
+    if (fd_force && (rc = alloc_zero(c, c.force, size_t(std::max(na, 1)) * 3 * kLanes * cb))) return rc;
     if (d.stats_every > 0 && (rc = alloc_zero(c, c.stats, size_t(std::max(na, 1)) * kStatFields * kLanes * 8))) return rc;
     if (d.stats_every > 0 && (rc = alloc_zero(c, c.stats_part, size_t(std::max(na, 1)) * kStatFields * kLanes * 4)))
         return rc;
@@ -527,7 +528,7 @@ int setup_device(lbm355_case& c)
     h.stats_epoch = 0;
     h.macro = c.macro.p;
     h.strain = c.strain.p;
-    h.force = nullptr;
+    h.force = fd_force ? c.force.p : nullptr;
     h.nbr = static_cast<const int32_t*>(c.nbr.p);
     h.label = static_cast<const uint8_t*>(c.label.p);
     h.bc_base = static_cast<const int32_t*>(c.bc_base.p);
@@ -580,6 +581,20 @@ int set_case_f64(lbm355_case& c, double* host_field, double v)
     uint64_t u;
     std::memcpy(&u, &v, 8);
     return set_case_u64(c, host_field, u);
+}
+
+StepLaunch single_launch(lbm355_case& c);
+
+// one time step of a case: the fd-force pre-kernels (velocity, strain, body
+// force) when that Carreau-Yasuda mode is on, then the fused step
+int launch_case_step(lbm355_case& c, const StepLaunch& L)
+{
+    if (c.host.force) {
+        const int r = launch_fd_force_prepass(c.storage, L, static_cast<const CaseDev*>(c.devcase.p), c.n_active,
+                                              c.stream);
+        if (r) return r;
+    }
+    return launch_step(c.storage, c.model, 0, L, c.stream);
 }
 
 StepLaunch single_launch(lbm355_case& c)
@@ -786,7 +801,7 @@ int lbm355_case_step(lbm355_case* c, int64_t n)
     StepLaunch L = single_launch(*c);
     for (int64_t s = 0; s < n; ++s) {
         L.rel = c->steps + s;
-        LAUNCH_TRY(launch_step(c->storage, c->model, 0, L, c->stream));
+        LAUNCH_TRY(launch_case_step(*c, L));
     }
     if (c->d.stats_every > 0) {
         const int64_t k = c->d.stats_every;
@@ -825,13 +840,13 @@ int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_step_ms, doub
     HIP_TRY(hipEventRecord(ev[2 * nk], c->stream));
     for (int64_t s = 0; s < n1; ++s) {
         L.rel = c->steps + s;
-        LAUNCH_TRY(launch_step(c->storage, c->model, 0, L, c->stream));
+        LAUNCH_TRY(launch_case_step(*c, L));
     }
     HIP_TRY(hipEventRecord(ev[2 * nk + 1], c->stream));
     for (int64_t s = 0; s < n2; ++s) {
         L.rel = c->steps + n1 + s;
         if (s < nk) HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
-        LAUNCH_TRY(launch_step(c->storage, c->model, 0, L, c->stream));
+        LAUNCH_TRY(launch_case_step(*c, L));
         if (s < nk) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -903,8 +918,12 @@ int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes)
         return f64 ? gather_dense<double>(*c, c->macro.p, 4, field, field == 0 ? 1.0 : 0.0, host)
                    : gather_dense<float>(*c, c->macro.p, 4, field, field == 0 ? 1.0 : 0.0, host);
     case LBM355_FIELD_FX: case LBM355_FIELD_FY: case LBM355_FIELD_FZ:
-        std::fill(host, host + N, 0.0);
-        return LBM355_OK;
+        if (!c->force.p) {
+            std::fill(host, host + N, 0.0);
+            return LBM355_OK;
+        }
+        return f64 ? gather_dense<double>(*c, c->force.p, 3, field - LBM355_FIELD_FX, 0.0, host)
+                   : gather_dense<float>(*c, c->force.p, 3, field - LBM355_FIELD_FX, 0.0, host);
     case LBM355_FIELD_LABEL:
         for (int64_t i = 0; i < N; ++i) host[i] = c->lab.map[i];
         return LBM355_OK;
@@ -1234,6 +1253,8 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
         if (!c) return fail(LBM355_EINVAL, "null case in batch");
         if (c->storage != b->storage || c->model != b->model)
             return fail(LBM355_EINVAL, "a batch needs one collision and one storage type");
+        if (c->host.force)
+            return fail(LBM355_EINVAL, "fd-force cases step alone (their pre-kernels are per case)");
         HIP_TRY(hipStreamSynchronize(c->stream));
         b->cases.push_back(c);
         cd.push_back(c->host);

@@ -120,7 +120,9 @@ int launch_copy(void* dst, const void* src, int64_t bytes, void* stream);
 int launch_advance(int64_t* base, int64_t k, void* stream);
 int launch_fill_populations(int storage, void* f, int64_t n_bricks, void* stream);
 int launch_strain(int storage, const CaseDev* dev_case, int n_active, void* stream);
-int launch_fd_force_prepass(int storage, const CaseDev* dev_case, int n_active, void* stream);
+// fd-force Carreau-Yasuda mode: velocity pre-pass, strain and body force of
+// the step L is about to run (launch them right before it on the same stream)
+int launch_fd_force_prepass(int storage, const StepLaunch& L, const CaseDev* dev_case, int n_active, void* stream);
 int launch_flux(int storage, const CaseDev* dev_case, int n_bc, double* dev_out, void* stream);
 int launch_set_u64(void* dev_ptr, uint64_t value, void* stream);
 int launch_zero(void* ptr, int64_t bytes, void* stream);

@@ -715,6 +715,54 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
     }
 }
 
+// velocity pre-pass of the fd-force Carreau-Yasuda mode (core.h:293-384):
+// rho and u = (sum c f + F/2) / rho of every fluid cell from the populations
+// its step is about to read (same AA addressing as step_brick) and the force
+// of the previous step, into the macro buffer
+template <int S>
+__device__ __forceinline__ void prepass_brick(const StepLaunch& L, const BrickRef br, int lane)
+{
+    using P = Pop<S>;
+    using R = typename P::R;
+    const CaseDev& C = L.cases[br.case_id];
+    const int b = int(br.brick);
+    const rsrc_t F = make_rsrc(C.f, C.f_bytes);
+    const int64_t step = *L.base + L.rel - C.start;
+    const int parity = int(step & 1);
+    const int lab = ((const __attribute__((address_space(1))) uint8_t*)C.label)[b * kLanes + lane];
+    if (lab != L_FLUID) return;
+    const __attribute__((address_space(4))) int32_t* nbr_c =
+        (const __attribute__((address_space(4))) int32_t*)C.nbr + b * 27;
+    int t[27];
+#pragma unroll
+    for (int q = 0; q < 27; ++q) t[q] = nbr_c[q];
+    const int i = lane & 3, j = (lane >> 2) & 3, k = lane >> 4;
+    R f[27];
+    if (parity == 0) {
+        const uint32_t own = uint32_t(t[13]) + uint32_t(lane);
+#pragma unroll
+        for (int q = 0; q < 27; ++q) f[q] = P::ld(F, own + uint32_t(q * kLanes), q);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const int cx = qx(q), cy = qy(q), cz = qz(q);
+            const int si = i - cx, sj = j - cy, sk = k - cz;
+            const int nb = pick_brick(t, -cx, -cy, -cz, (si & ~3) != 0, (sj & ~3) != 0, (sk & ~3) != 0);
+            const int l2 = (si & 3) | ((sj & 3) << 2) | ((sk & 3) << 4);
+            f[q] = P::ld(F, uint32_t(nb + (26 - q) * kLanes + l2), q);
+        }
+    }
+    const R* fo = static_cast<const R*>(C.force);
+    const R Fx = fo[(b * 3 + 0) * kLanes + lane], Fy = fo[(b * 3 + 1) * kLanes + lane];
+    const R Fz = fo[(b * 3 + 2) * kLanes + lane];
+    const Moments<R> M = moments2<R>(f);
+    R* mo = static_cast<R*>(C.macro);
+    mo[(b * 4 + 0) * kLanes + lane] = M.rho;
+    mo[(b * 4 + 1) * kLanes + lane] = (M.jx + R(0.5) * Fx) / M.rho;
+    mo[(b * 4 + 2) * kLanes + lane] = (M.jy + R(0.5) * Fy) / M.rho;
+    mo[(b * 4 + 3) * kLanes + lane] = (M.jz + R(0.5) * Fz) / M.rho;
+}
+
 // XCD-aware mapping: workgroups go round-robin to the 8 XCDs; remap them so
 // that each XCD walks one contiguous (Morton-ordered, hence spatially compact)
 // chunk of the brick list and its L2 sees the neighbour bricks.
@@ -740,6 +788,79 @@ __global__ void __launch_bounds__(512) k_step(StepLaunch L)
         const BrickRef br = L.list[w];
         step_brick<S, MODEL>(L, br, int(threadIdx.x) & 63);
     }
+}
+
+template <int S>
+__global__ void __launch_bounds__(64) k_prepass(StepLaunch L)
+{
+    if (int(blockIdx.x) < L.n_list) prepass_brick<S>(L, L.list[blockIdx.x], int(threadIdx.x));
+}
+
+// body force of the fd-force mode (core.h:663-781) from the strain of the
+// pre-pass velocities: F_c = sum_a d_a S_{a c} with the one-sided / central /
+// zero rule of the fluid neighbours, nu from the cell's own strain in single
+// precision as the reference's powf (core.h:766), and
+// F = 2 (nu - nu_ref) F_c rho with rho still 1 at that point of the reference
+// kernel.  Non-fluid cells get no force.
+template <class R>
+__global__ void __launch_bounds__(64) k_force(const CaseDev* Cp)
+{
+    const CaseDev& C = *Cp;
+    const int b = int(blockIdx.x);
+    const int lane = int(threadIdx.x);
+    __shared__ R ss[6][216];
+    __shared__ uint8_t sf[216];
+    const R* so = static_cast<const R*>(C.strain);
+    for (int c = lane; c < 216; c += 64) {
+        const int hx = c % 6, hy = (c / 6) % 6, hz = c / 36;
+        const int gx = hx - 1, gy = hy - 1, gz = hz - 1;
+        const int ox = (gx < 0) ? -1 : (gx > 3 ? 1 : 0);
+        const int oy = (gy < 0) ? -1 : (gy > 3 ? 1 : 0);
+        const int oz = (gz < 0) ? -1 : (gz > 3 ? 1 : 0);
+        const int nb = C.nbr[b * 27 + 13 + 9 * ox + 3 * oy + oz] / (27 * kLanes);
+        const int l = (gx & 3) | ((gy & 3) << 2) | ((gz & 3) << 4);
+        const bool fl = C.label[nb * kLanes + l] == L_FLUID && nb < C.n_active;
+        sf[c] = fl ? 1 : 0;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) ss[m][c] = fl ? so[(nb * 6 + m) * kLanes + l] : R(0);
+    }
+    __syncthreads();
+    const int i = lane & 3, j = (lane >> 2) & 3, k = lane >> 4;
+    const int c0 = (i + 1) + 6 * ((j + 1) + 6 * (k + 1));
+    R Fo[3] = {R(0), R(0), R(0)};
+    if (sf[c0]) {
+        // S components: 0 S11, 1 S12, 2 S13, 3 S22, 4 S32, 5 S33; rows of the
+        // derivative along x, y, z
+        const int rows[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        const int st[3] = {1, 6, 36};
+        R Fc[3] = {R(0), R(0), R(0)};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int cm = c0 - st[a], cp = c0 + st[a];
+            const bool m = sf[cm], p = sf[cp];
+            if (!m && !p) continue;
+#pragma unroll
+            for (int comp = 0; comp < 3; ++comp) {
+                const R* v = ss[rows[a][comp]];
+                if (!m) Fc[comp] += v[cp] - v[c0];
+                else if (!p) Fc[comp] += v[c0] - v[cm];
+                else Fc[comp] += R(0.5) * (v[cp] - v[cm]);
+            }
+        }
+        const R s11 = ss[0][c0], s12 = ss[1][c0], s13 = ss[2][c0], s22 = ss[3][c0], s32 = ss[4][c0], s33 = ss[5][c0];
+        const R gam = sqrt(R(2)) * sqrt(s11 * s11 + s22 * s22 + s33 * s33 + R(2) * (s12 * s12 + s13 * s13 + s32 * s32));
+        // single-precision pow as the correctly rounded float of the double
+        // pow (the oracle evaluates the same expression)
+        auto powf_cr = [](float x, float y) { return float(pow(double(x), double(y))); };
+        const float inner = powf_cr(float(gam * R(C.cy_lambda)), float(C.cy_a));
+        const R nu = R(C.nu_ref) + (R(C.cy_nu0) - R(C.nu_ref)) *
+                                       R(powf_cr(float(1.0 + double(inner)), float((C.cy_n - 1.0) / C.cy_a)));
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp) Fo[comp] = R(2) * (nu - R(C.nu_ref)) * Fc[comp] * R(1);
+    }
+    R* fo = static_cast<R*>(C.force);
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) fo[(b * 3 + comp) * kLanes + lane] = Fo[comp];
 }
 
 template <int S>
@@ -992,10 +1113,23 @@ int launch_advance(int64_t* base, int64_t k, void* stream)
     return check_launch();
 }
 
-int launch_fd_force_prepass(int, const CaseDev*, int, void*)
+int launch_fd_force_prepass(int storage, const StepLaunch& L, const CaseDev* dev_case, int n_active, void* stream)
 {
-    g_kerr = "fd-force coupling is not available in this build";
-    return -1;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (n_active <= 0) return 0;
+    switch (storage) {
+    case S_F64: hipLaunchKernelGGL(k_prepass<S_F64>, dim3(n_active), dim3(64), 0, st, L); break;
+    case S_F32: hipLaunchKernelGGL(k_prepass<S_F32>, dim3(n_active), dim3(64), 0, st, L); break;
+    case S_F16: hipLaunchKernelGGL(k_prepass<S_F16>, dim3(n_active), dim3(64), 0, st, L); break;
+    case S_BF16: hipLaunchKernelGGL(k_prepass<S_BF16>, dim3(n_active), dim3(64), 0, st, L); break;
+    default: g_kerr = "unknown storage"; return -1;
+    }
+    int rc = check_launch();
+    if (rc) return rc;
+    if ((rc = launch_strain(storage, dev_case, n_active, stream))) return rc;
+    if (storage == S_F64) hipLaunchKernelGGL(k_force<double>, dim3(n_active), dim3(64), 0, st, dev_case);
+    else hipLaunchKernelGGL(k_force<float>, dim3(n_active), dim3(64), 0, st, dev_case);
+    return check_launch();
 }
 
 }  // namespace lbm355

@@ -104,10 +104,12 @@ void lbm355_pool::run()
 {
     (void)hipSetDevice(device);
     std::vector<lbm355_batch*> batches;
+    std::vector<lbm355_case*> solo;  // fd-force cases: their pre-kernels are per case
     bool rebuild = false;
     auto drop_batches = [&]() {
         for (lbm355_batch* b : batches) lbm355_batch_destroy(b);
         batches.clear();
+        solo.clear();
     };
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
@@ -162,7 +164,10 @@ void lbm355_pool::run()
         if (rebuild) {
             drop_batches();
             std::map<std::pair<int, int>, std::vector<lbm355_case*>> groups;
-            for (Request* r : running) groups[{r->desc.storage, r->desc.collision}].push_back(r->c);
+            for (Request* r : running) {
+                if (r->desc.cy_enabled && r->desc.cy_model == LBM355_CY_FD_FORCE) solo.push_back(r->c);
+                else groups[{r->desc.storage, r->desc.collision}].push_back(r->c);
+            }
             int rc = LBM355_OK;
             for (auto& g : groups) {
                 lbm355_batch* b = nullptr;
@@ -197,8 +202,14 @@ void lbm355_pool::run()
         int rc = LBM355_OK;
         for (lbm355_batch* b : batches)
             if (rc == LBM355_OK) rc = batch_enqueue(b, n);
+        for (lbm355_case* c : solo)
+            if (rc == LBM355_OK) rc = lbm355_case_step(c, n);  // queued on the case's stream
         for (lbm355_batch* b : batches) {
             const int r2 = batch_finish(b);
+            if (rc == LBM355_OK) rc = r2;
+        }
+        for (lbm355_case* c : solo) {
+            const int r2 = lbm355_case_sync(c);
             if (rc == LBM355_OK) rc = r2;
         }
         const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_tick).count();

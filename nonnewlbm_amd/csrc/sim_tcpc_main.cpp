@@ -91,7 +91,9 @@ void usage()
               << "  --material cy-a|cy-b|blood-newtonian   benchmark materials\n"
               << "  --collision cumulant|bgk|mrt|kbc       default cumulant\n"
               << "  --precision fp32|fp64|fp16|bf16        population storage, default fp32\n"
-              << "  --cy-model neq|off                     Carreau-Yasuda coupling, default neq\n"
+              << "  --cy-model neq|off|fd-force            Carreau-Yasuda coupling, default neq; off = what a\n"
+              << "                                         single-rank reference run computes, fd-force = the\n"
+              << "                                         reference's multi-rank algorithm\n"
               << "  --steps N                              run exactly N steps (overrides --final-time)\n"
               << "  --inflow-profile plug|parabolic        default plug (u = flux / area)\n"
               << "  --ivc-umax V --svc-umax V              parabolic peak velocities in m/s\n"

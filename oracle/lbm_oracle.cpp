@@ -716,10 +716,15 @@ struct Oracle : OracleBase {
                         T s11 = S[0][i], s12 = S[1][i], s13 = S[2][i], s22 = S[3][i], s32 = S[4][i], s33 = S[5][i];
                         T gam = std::sqrt(T(2)) * std::sqrt(s11 * s11 + s22 * s22 + s33 * s33 +
                                                             T(2) * (s12 * s12 + s13 * s13 + s32 * s32));
-                        // the reference evaluates nu with powf (core.h:766)
+                        // the reference evaluates nu in single precision with powf
+                        // (core.h:766); powf here is the correctly rounded float of
+                        // the double pow, as on the device (the reference's own
+                        // builds differ: glibc powf on the host, fast-math __powf
+                        // in CUDA), so that the two agree bit for bit
+                        auto powf_cr = [](float x, float y) { return float(std::pow(double(x), double(y))); };
                         T nu = T(d.nu_ref) + (T(d.cy_nu0) - T(d.nu_ref)) *
-                               T(powf(float(1.0 + powf(float(gam * T(d.cy_lambda)), float(d.cy_a))),
-                                      float((d.cy_n - 1.0) / d.cy_a)));
+                               T(powf_cr(float(1.0 + powf_cr(float(gam * T(d.cy_lambda)), float(d.cy_a))),
+                                         float((d.cy_n - 1.0) / d.cy_a)));
                         k.fx += T(2) * (nu - T(d.nu_ref)) * F[0] * k.rho;
                         k.fy += T(2) * (nu - T(d.nu_ref)) * F[1] * k.rho;
                         k.fz += T(2) * (nu - T(d.nu_ref)) * F[2] * k.rho;

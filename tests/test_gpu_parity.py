@@ -160,3 +160,71 @@ def test_carreau_yasuda_neq(gpu_lib, storage, collision):
     tol = 1e-8 if storage == _abi.F64 else 1e-3
     for name in ("rho", "vx", "vy", "vz"):
         assert rel_l2(gpu.field(name), cpu.field(name), act) < tol, name
+
+
+@pytest.mark.parametrize("profile", [_abi.PROFILE_PLUG, _abi.PROFILE_PARABOLIC])
+def test_pulsatile_inflow(gpu_lib, profile):
+    """u_in(t) = u (1 + A sin(2 pi t / T)), t the step index (the per-step
+    inflow hook, sim_tcpc.cu:657-663, driven inside the kernel)."""
+    g, kw = tcpc_case(0.25, storage=_abi.F64, pulse_amp=0.3, pulse_period=40.0, profile=profile,
+                      ivc_umax=0.04, svc_umax=0.03, ivc_center=(-1.0, -1.0), svc_center=(-1.0, -1.0))
+    gpu, cpu = _pair(g, kw)
+    act = _active(cpu)
+    lab = cpu.field("label")
+    for steps in (1, 2, 7):
+        gpu.step(steps)
+        cpu.step(steps)
+        pg, pc = gpu.populations(), cpu.populations()
+        stored = pg[:, act] != 0.0
+        assert np.max(np.abs(pg - pc)[:, act][stored]) < 1e-12
+    # the inlet velocity follows the pulse: step 10 (index 9) of period 40
+    vx = gpu.field("vx")[lab == 3]
+    gpu2 = Case(g.flat(), g.dims, **dict(kw, pulse_amp=0.0))
+    gpu2.step(10)
+    np.testing.assert_allclose(vx, gpu2.field("vx")[lab == 3] * (1 + 0.3 * np.sin(2 * np.pi * 9 / 40)), rtol=1e-12)
+    gpu.step(90)
+    cpu.step(90)
+    for name in ("rho", "vx", "vy", "vz"):
+        assert rel_l2(gpu.field(name), cpu.field(name), act) < 1e-9, name
+
+
+@pytest.mark.parametrize("storage", [_abi.F64, _abi.F32])
+def test_carreau_yasuda_fd_force(gpu_lib, storage):
+    """The reference's multi-rank Carreau-Yasuda algorithm (core.h:293-384,
+    407-576, 663-781): velocity pre-pass, FD strain, body force.  nu is
+    evaluated in single precision as the reference's powf, on both sides as the
+    correctly rounded float of the double pow."""
+    from nonnewlbm_amd.materials import case_kwargs, material_by_name
+    g, kw = tcpc_case(0.25, storage=storage)
+    kw.update(case_kwargs(material_by_name("sx"), g.dl, 0.02, cy_model=_abi.CY_FD_FORCE))
+    gpu = Case(g.flat(), g.dims, **kw)
+    cpu = OracleCase(g.flat(), g.dims, **dict(kw, storage=_abi.F64))
+    act = _active(cpu)
+    fl = cpu.field("label") == 0
+    gpu.step(2)
+    cpu.step(2)
+    tol = 1e-12 if storage == _abi.F64 else 1e-3
+    for name in ("s11", "s12", "s33", "fx", "fy", "fz"):
+        a, b = gpu.field(name)[fl], cpu.field(name)[fl]
+        assert np.max(np.abs(a - b)) <= tol * max(np.max(np.abs(b)), 1e-30), name
+    assert np.abs(cpu.field("fx")[fl]).max() > 0
+    gpu.step(100)
+    cpu.step(100)
+    tol = 1e-10 if storage == _abi.F64 else 1e-3
+    for name in ("rho", "vx", "vy", "vz", "fx"):
+        assert rel_l2(gpu.field(name), cpu.field(name), act if name != "fx" else fl) < tol, name
+
+
+def test_output_strain_matches_oracle(gpu_lib):
+    """The output-path strain (cudaLBMKernelStress stencil, core.h:407-576)
+    of the final velocity field."""
+    g, kw = tcpc_case(0.25, storage=_abi.F64)
+    gpu, cpu = _pair(g, kw)
+    gpu.step(30)
+    cpu.step(30)
+    gpu.compute_strain()
+    cpu.compute_strain()
+    fl = cpu.field("label") == 0
+    for name in ("s11", "s12", "s13", "s22", "s32", "s33"):
+        np.testing.assert_allclose(gpu.field(name), cpu.field(name), rtol=0, atol=1e-13, err_msg=name)
+    assert np.abs(cpu.field("s12")[fl]).max() > 1e-6

@@ -20,9 +20,12 @@ def _specs():
                                                  (0.3, _abi.F16_OFFSET, _abi.KBC, 33, None),
                                                  (0.25, _abi.F32, _abi.CUM, 41, "cy-a"),
                                                  (0.3, _abi.F32, _abi.BGK, 57, None),
-                                                 (0.25, _abi.BF16_OFFSET, _abi.MRT, 20, None)):
+                                                 (0.25, _abi.BF16_OFFSET, _abi.MRT, 20, None),
+                                                 (0.25, _abi.F32, _abi.CUM, 29, "sx-fd")):
         g, kw = tcpc_case(scale, storage=storage, collision=collision)
-        if cy:
+        if cy == "sx-fd":   # the reference's fd-force mode steps outside the batches
+            kw.update(case_kwargs(material_by_name("sx"), g.dl, 0.02, cy_model=_abi.CY_FD_FORCE))
+        elif cy:
             kw.update(case_kwargs(material_by_name(cy), g.dl, 0.01))
         out.append((g, kw, steps))
     return out
