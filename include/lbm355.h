@@ -195,7 +195,8 @@ int lbm355_case_set_shape(lbm355_case* c, int bricks_per_group);
 int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes);
 /* populations after the last step in the reference's A/B sense: out[q][cell] =
    post-collision f_q of each active cell (df_out after SimUpdate); wall cells
-   are 0; bytes >= 27*nx*ny*nz*8 */
+   next to the flow carry the populations in flight through full-way
+   bounce-back, other wall cells are 0; bytes >= 27*nx*ny*nz*8 */
 int lbm355_case_get_populations(lbm355_case* c, double* host, size_t bytes);
 
 /* finite-difference strain S of the current velocity field with the reference
@@ -223,6 +224,32 @@ typedef struct lbm355_output_desc {
     int32_t reserved[6];
 } lbm355_output_desc;
 int lbm355_case_write_vtk(lbm355_case* c, const char* path, const lbm355_output_desc* o);
+
+/* --- checkpoint / restart ------------------------------------------------
+ * The reference's current_state layout (saveAndLoadState, state.hpp:1552-1640):
+ * <dir>/config_rank000 (iterations, X, Y, Z, local X, Y, Z, final time),
+ * cnt_<k>_rank000 for the 11 output counters (count, period; state.h:72),
+ * df_0_rank000 and df_1_rank000 (populations in the A/B sense, both the same;
+ * the next step reads df_(iterations % 2)), map_rank000 and macro_rank000.
+ * Arrays are raw binary in the index order of the reference's GPU build
+ * (defs.h:57-60): df[x][q][z][y] and macro[x][id][z][y] in the real type of the
+ * storage (float; double for LBM355_F64), map[x][z][y] as int16; macro ids are
+ * the 20 fields of MacroTcpc (sim_tcpc.cu:328-335).  lbm355_rank000 adds what
+ * the reference does not keep (the statistics reset step and sample count).
+ * A restart needs a case created from the same descriptor; run N steps, save,
+ * load, run M equals N + M steps. */
+typedef struct lbm355_state_desc {
+    double phys_final_time;        /* config line 8                                 */
+    int32_t counter_count[11];     /* STAT_RESET, STAT2_RESET, PRINT, VTK1D, VTK2D,
+                                      VTK3D, PROBE1, PROBE2, PROBE3, SAVESTATE,
+                                      VTK3DCUT                                      */
+    double counter_period[11];
+    int64_t stat_prev_iter;        /* step of the last statistics reset              */
+    int64_t iterations;            /* set by load                                    */
+    int32_t reserved[4];
+} lbm355_state_desc;
+int lbm355_case_save_state(lbm355_case* c, const char* dir, const lbm355_state_desc* s);
+int lbm355_case_load_state(lbm355_case* c, const char* dir, lbm355_state_desc* s);
 
 /* --- batches: many cases stepped by one launch per step ------------------ */
 /* No counterpart in the reference (it runs one case per process,

@@ -84,6 +84,14 @@ class CaseInfo(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+class StateDesc(C.Structure):
+    _fields_ = [
+        ("phys_final_time", C.c_double),
+        ("counter_count", C.c_int32 * 11), ("counter_period", C.c_double * 11),
+        ("stat_prev_iter", C.c_int64), ("iterations", C.c_int64), ("reserved", C.c_int32 * 4),
+    ]
+
+
 class OutputDesc(C.Structure):
     _fields_ = [
         ("phys_density", C.c_double), ("phys_nu_inf", C.c_double), ("time", C.c_double),
@@ -205,6 +213,8 @@ def load() -> C.CDLL:
         "lbm355_case_labels": (C.c_int, [C.POINTER(CaseDesc), C.POINTER(C.c_int32), C.POINTER(CaseInfo)]),
         "lbm355_case_time_kernel": (C.c_int, [P, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "lbm355_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, C.POINTER(C.c_double)]),
+        "lbm355_case_save_state": (C.c_int, [P, C.c_char_p, C.POINTER(StateDesc)]),
+        "lbm355_case_load_state": (C.c_int, [P, C.c_char_p, C.POINTER(StateDesc)]),
         "lbm355_pool_create": (C.c_int, [C.c_int64, C.c_int, C.c_int, C.POINTER(P)]),
         "lbm355_pool_submit": (C.c_int, [P, C.POINTER(CaseDesc), C.c_int64, C.c_int64, C.c_int,
                                          C.POINTER(C.c_int64)]),
@@ -232,6 +242,7 @@ EXPORTED = [
     "lbm355_batch_create", "lbm355_batch_step", "lbm355_batch_destroy", "lbm355_batch_set_shape",
     "lbm355_batch_autotune", "lbm355_case_footprint", "lbm355_case_labels", "lbm355_case_time_kernel",
     "lbm355_copy_bandwidth",
+    "lbm355_case_save_state", "lbm355_case_load_state",
     "lbm355_pool_create", "lbm355_pool_submit", "lbm355_pool_poll", "lbm355_pool_wait", "lbm355_pool_drain",
     "lbm355_pool_take", "lbm355_pool_stats", "lbm355_pool_destroy",
 ]

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <sys/stat.h>
 #include <memory>
 #include <new>
 #include <string>
@@ -199,8 +200,9 @@ struct lbm355_case {
     double inflow[2] = {0, 0};
     int64_t steps = 0;
     int64_t macro_valid_step = -1;      // step index whose rho,u are in the macro buffer
-    int64_t stats_steps = 0;            // steps accumulated since the last reset
-    int64_t stats_first = 0;            // step index of the last reset (the epoch)
+    int64_t stats_steps = 0;            // samples accumulated since the last reset
+    int64_t stats_first = 0;            // step index of the epoch (last reset or restart)
+    int64_t stats_block0 = 0;           // stats_steps at the epoch: FP32 blocks count from there
     bool strain_valid = false;
 
     hipStream_t stream = nullptr;
@@ -636,6 +638,30 @@ int gather_dense(lbm355_case& c, const void* dev, int nf, int field, double fill
     return LBM355_OK;
 }
 
+// inverse of gather_dense: overwrite the entries of the active cells of one
+// field of a per-active-brick device array from a dense double array
+template <class R>
+int scatter_dense(lbm355_case& c, void* dev, int nf, int field, const double* in)
+{
+    const int na = c.n_active;
+    if (na <= 0) return LBM355_OK;
+    std::vector<R> h(size_t(na) * nf * kLanes);
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    HIP_TRY(hipMemcpy(h.data(), dev, h.size() * sizeof(R), hipMemcpyDeviceToHost));
+    const int X = c.d.nx, Y = c.d.ny, Z = c.d.nz;
+    for (int b = 0; b < na; ++b)
+        for (int l = 0; l < kLanes; ++l) {
+            int x, y, z;
+            c.cell_of(b, l, x, y, z);
+            if (x < 0 || y < 0 || x >= X || y >= Y || z >= Z) continue;
+            const int64_t i = c.cell_index(x, y, z);
+            if (c.lab.map[i] == L_WALL) continue;
+            h[(size_t(b) * nf + field) * kLanes + l] = R(in[i]);
+        }
+    HIP_TRY(hipMemcpy(dev, h.data(), h.size() * sizeof(R), hipMemcpyHostToDevice));
+    return LBM355_OK;
+}
+
 int refresh_macro(lbm355_case& c)
 {
     if (c.macro_valid_step == c.steps - 1 || c.steps == 0) return LBM355_OK;
@@ -935,7 +961,7 @@ int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes)
     if (field >= LBM355_FIELD_SUM_RUX && field <= LBM355_FIELD_SUM_RRUZZ) {
         if (c->d.stats_every <= 0) { std::fill(host, host + N, 0.0); return LBM355_OK; }
         if ((rc = gather_dense<double>(*c, c->stats.p, kStatFields, field - LBM355_FIELD_SUM_RUX, 0.0, host))) return rc;
-        if (!f64 && c->stats_steps % kStatBlock != 0) {
+        if (!f64 && (c->stats_steps - c->stats_block0) % kStatBlock != 0) {
             // samples of the open FP32 block are not in the FP64 sums yet
             std::vector<double> part(static_cast<size_t>(N));
             if ((rc = gather_dense<float>(*c, c->stats_part.p, kStatFields, field - LBM355_FIELD_SUM_RUX, 0.0,
@@ -994,12 +1020,15 @@ int lbm355_case_get_populations(lbm355_case* c, double* host, size_t bytes)
         for (int y = 0; y < Y; ++y)
             for (int x = 0; x < X; ++x) {
                 const int64_t i = c->cell_index(x, y, z);
-                if (c->lab.map[i] == L_WALL) continue;
+                // wall cells next to the flow hold the populations in flight
+                // through full-way bounce-back; the others hold nothing
+                if (c->lab.map[i] == L_WALL && c->stored_brick(x, y, z) < 0) continue;
                 for (int q = 0; q < kQ; ++q) {
                     int32_t off;
                     if (even_next) {
                         const int tx = x + qx(q), ty = y + qy(q), tz = z + qz(q);
                         if (tx < 0 || ty < 0 || tz < 0 || tx >= X || ty >= Y || tz >= Z) continue;
+                        if (c->stored_brick(tx, ty, tz) < 0) continue;
                         off = c->slot_offset(tx, ty, tz, q);
                     } else {
                         off = c->slot_offset(x, y, z, 26 - q);
@@ -1038,7 +1067,7 @@ int lbm355_case_flux(lbm355_case* c, int mean, double out[4])
     std::vector<double> h(size_t(std::max(na, 1)) * kStatFields * kLanes);
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (na > 0) HIP_TRY(hipMemcpy(h.data(), c->stats.p, h.size() * 8, hipMemcpyDeviceToHost));
-    if (c->storage != LBM355_F64 && c->stats_steps % kStatBlock != 0) {
+    if (c->storage != LBM355_F64 && (c->stats_steps - c->stats_block0) % kStatBlock != 0) {
         // the open FP32 block of samples
         std::vector<float> part(h.size());
         if (na > 0) HIP_TRY(hipMemcpy(part.data(), c->stats_part.p, part.size() * 4, hipMemcpyDeviceToHost));
@@ -1073,7 +1102,302 @@ int lbm355_case_reset_stats(lbm355_case* c)
         if ((rc = set_case_u64(*c, &c->host.stats_epoch, uint64_t(c->steps)))) return rc;
     }
     c->stats_steps = 0;
+    c->stats_block0 = 0;
     c->stats_first = c->steps;
+    return LBM355_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// checkpoint / restart in the reference's current_state layout
+namespace {
+
+const int kCounters = 11;  // STAT_RESET ... VTK3DCUT (state.h:72)
+const int kMacroTcpc = 20; // MacroTcpc::N (sim_tcpc.cu:328-335)
+
+std::string state_file(const char* dir, const char* name)
+{
+    return std::string(dir) + "/" + name + "_rank000";
+}
+
+int make_dirs(const std::string& path)
+{
+    std::string cur;
+    for (size_t i = 0; i <= path.size(); ++i) {
+        if (i == path.size() || path[i] == '/') {
+            if (!cur.empty()) ::mkdir(cur.c_str(), 0777);
+        }
+        if (i < path.size()) cur.push_back(path[i]);
+    }
+    struct stat st;
+    return ::stat(path.c_str(), &st) == 0 && S_ISDIR(st.st_mode) ? 0 : -1;
+}
+
+template <class T>
+int write_raw(const std::string& f, const std::vector<T>& v)
+{
+    FILE* fp = std::fopen(f.c_str(), "wb");
+    if (!fp) return fail(LBM355_EINVAL, "cannot create " + f);
+    const size_t n = std::fwrite(v.data(), sizeof(T), v.size(), fp);
+    std::fclose(fp);
+    return n == v.size() ? LBM355_OK : fail(LBM355_EINVAL, "short write " + f);
+}
+
+template <class T>
+int read_raw(const std::string& f, std::vector<T>& v)
+{
+    FILE* fp = std::fopen(f.c_str(), "rb");
+    if (!fp) return fail(LBM355_EINVAL, "cannot open " + f);
+    const size_t n = std::fread(v.data(), sizeof(T), v.size(), fp);
+    std::fclose(fp);
+    return n == v.size() ? LBM355_OK : fail(LBM355_EINVAL, "short read " + f);
+}
+
+// macro field ids of MacroTcpc -> our dense fields (-1: zero; e_vm_minus_x is
+// kept inside e_vm_plus_x, only their sum is read, sim_tcpc.cu:724-725)
+const int kMacroMap[kMacroTcpc] = {
+    LBM355_FIELD_RHO, LBM355_FIELD_VX, LBM355_FIELD_VY, LBM355_FIELD_VZ,
+    LBM355_FIELD_SUM_RUX, -1, LBM355_FIELD_SUM_RUY, LBM355_FIELD_SUM_RUZ,
+    LBM355_FIELD_SUM_RRUXX, LBM355_FIELD_SUM_RRUYY, LBM355_FIELD_SUM_RRUZZ,
+    LBM355_FIELD_FX, LBM355_FIELD_FY, LBM355_FIELD_FZ,
+    LBM355_FIELD_S11, LBM355_FIELD_S12, LBM355_FIELD_S13, LBM355_FIELD_S22, LBM355_FIELD_S32, LBM355_FIELD_S33};
+
+template <class T>
+int save_arrays(lbm355_case* c, const char* dir, const std::vector<double>& pops)
+{
+    const int64_t X = c->d.nx, Y = c->d.ny, Z = c->d.nz, N = X * Y * Z;
+    // df: the reference's lat layout x, q, z, y (y fastest, defs.h:57-58)
+    std::vector<T> df(size_t(N) * kQ);
+    size_t k = 0;
+    for (int64_t x = 0; x < X; ++x)
+        for (int q = 0; q < kQ; ++q)
+            for (int64_t z = 0; z < Z; ++z)
+                for (int64_t y = 0; y < Y; ++y) df[k++] = T(pops[size_t(q * N + x + X * (y + Y * z))]);
+    int rc;
+    // both buffers hold the current populations; the one the next step
+    // reads is df_(iterations % 2)
+    if ((rc = write_raw(state_file(dir, "df_0"), df))) return rc;
+    if ((rc = write_raw(state_file(dir, "df_1"), df))) return rc;
+    std::vector<int16_t> map(static_cast<size_t>(N));
+    k = 0;
+    for (int64_t x = 0; x < X; ++x)
+        for (int64_t z = 0; z < Z; ++z)
+            for (int64_t y = 0; y < Y; ++y) map[k++] = int16_t(c->lab.map[size_t(x + X * (y + Y * z))]);
+    if ((rc = write_raw(state_file(dir, "map"), map))) return rc;
+    std::vector<double> f(static_cast<size_t>(N));
+    std::vector<T> macro(static_cast<size_t>(N) * kMacroTcpc, T(0));
+    for (int id = 0; id < kMacroTcpc; ++id) {
+        if (kMacroMap[id] < 0) continue;
+        if ((rc = lbm355_case_get_field(c, kMacroMap[id], f.data(), f.size() * 8))) return rc;
+        for (int64_t x = 0; x < X; ++x)
+            for (int64_t z = 0; z < Z; ++z)
+                for (int64_t y = 0; y < Y; ++y)
+                    macro[size_t(y + Y * (z + Z * (id + int64_t(kMacroTcpc) * x)))] = T(f[size_t(x + X * (y + Y * z))]);
+    }
+    return write_raw(state_file(dir, "macro"), macro);
+}
+
+template <class T>
+int load_arrays(lbm355_case* c, const char* dir, int64_t iterations, std::vector<double>& pops,
+                std::vector<std::vector<double>>& fields)
+{
+    const int64_t X = c->d.nx, Y = c->d.ny, Z = c->d.nz, N = X * Y * Z;
+    std::vector<T> df(size_t(N) * kQ);
+    int rc;
+    if ((rc = read_raw(state_file(dir, iterations % 2 ? "df_1" : "df_0"), df))) return rc;
+    pops.assign(size_t(N) * kQ, 0.0);
+    size_t k = 0;
+    for (int64_t x = 0; x < X; ++x)
+        for (int q = 0; q < kQ; ++q)
+            for (int64_t z = 0; z < Z; ++z)
+                for (int64_t y = 0; y < Y; ++y) pops[size_t(q * N + x + X * (y + Y * z))] = double(df[k++]);
+    std::vector<T> macro(size_t(N) * kMacroTcpc);
+    if ((rc = read_raw(state_file(dir, "macro"), macro))) return rc;
+    fields.assign(kMacroTcpc, std::vector<double>());
+    for (int id = 0; id < kMacroTcpc; ++id) {
+        fields[id].assign(size_t(N), 0.0);
+        for (int64_t x = 0; x < X; ++x)
+            for (int64_t z = 0; z < Z; ++z)
+                for (int64_t y = 0; y < Y; ++y)
+                    fields[id][size_t(x + X * (y + Y * z))] =
+                        double(macro[size_t(y + Y * (z + Z * (id + int64_t(kMacroTcpc) * x)))]);
+    }
+    // the AA-split x statistic is their sum
+    for (int64_t i = 0; i < N; ++i) fields[4][size_t(i)] += fields[5][size_t(i)];
+    return LBM355_OK;
+}
+
+uint16_t host_f16(float v)
+{
+    const _Float16 h = static_cast<_Float16>(v);  // round to nearest even
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+
+uint16_t host_bf16(float v)
+{
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return uint16_t(u >> 16);
+}
+
+// write dense A/B-sense populations (as lbm355_case_get_populations returns
+// them) into the AA buffer for the case's current step parity
+int set_populations(lbm355_case* c, const std::vector<double>& pops)
+{
+    const int X = c->d.nx, Y = c->d.ny, Z = c->d.nz;
+    const int64_t N = int64_t(X) * Y * Z;
+    const size_t sb = storage_bytes(c->storage);
+    const size_t ne = size_t(c->n_stored + 1) * kQ * kLanes;
+    std::vector<unsigned char> raw(ne * sb);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(raw.data(), c->f.p, raw.size(), hipMemcpyDeviceToHost));
+    auto put = [&](int32_t off, int q, double v) {
+        switch (c->storage) {
+        case LBM355_F64: std::memcpy(&raw[size_t(off) * 8], &v, 8); break;
+        case LBM355_F32: { const float f = float(v); std::memcpy(&raw[size_t(off) * 4], &f, 4); break; }
+        case LBM355_F16_OFFSET: {
+            const uint16_t h = host_f16(float(v) - float(lattice_weight(q)));
+            std::memcpy(&raw[size_t(off) * 2], &h, 2);
+            break;
+        }
+        default: {
+            const uint16_t h = host_bf16(float(v) - float(lattice_weight(q)));
+            std::memcpy(&raw[size_t(off) * 2], &h, 2);
+        }
+        }
+    };
+    const bool even_next = (c->steps & 1) == 0;
+    for (int z = 0; z < Z; ++z)
+        for (int y = 0; y < Y; ++y)
+            for (int x = 0; x < X; ++x) {
+                const int64_t i = c->cell_index(x, y, z);
+                if (c->lab.map[i] == L_WALL && c->stored_brick(x, y, z) < 0) continue;
+                for (int q = 0; q < kQ; ++q) {
+                    int32_t off;
+                    if (even_next) {
+                        const int tx = x + qx(q), ty = y + qy(q), tz = z + qz(q);
+                        if (tx < 0 || ty < 0 || tz < 0 || tx >= X || ty >= Y || tz >= Z) continue;
+                        if (c->stored_brick(tx, ty, tz) < 0) continue;
+                        off = c->slot_offset(tx, ty, tz, q);
+                    } else {
+                        off = c->slot_offset(x, y, z, 26 - q);
+                    }
+                    put(off, q, pops[size_t(int64_t(q) * N + i)]);
+                }
+            }
+    HIP_TRY(hipMemcpy(c->f.p, raw.data(), raw.size(), hipMemcpyHostToDevice));
+    return LBM355_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbm355_case_save_state(lbm355_case* c, const char* dir, const lbm355_state_desc* s)
+{
+    if (!c || !dir || !s) return fail(LBM355_EINVAL, "null argument");
+    if (make_dirs(dir) != 0) return fail(LBM355_EINVAL, std::string("cannot create ") + dir);
+    const int64_t X = c->d.nx, Y = c->d.ny, Z = c->d.nz, N = X * Y * Z;
+    std::vector<double> pops(size_t(N) * kQ);
+    int rc;
+    if ((rc = lbm355_case_get_populations(c, pops.data(), pops.size() * 8))) return rc;
+    // config and counters as saveLoadTextData writes them (state.hpp:1478-1500, state.h:236-250)
+    FILE* fp = std::fopen(state_file(dir, "config").c_str(), "wt");
+    if (!fp) return fail(LBM355_EINVAL, "cannot create config");
+    std::fprintf(fp, "%d\n%ld\n%ld\n%ld\n%ld\n%ld\n%ld\n%le\n", int(c->steps), long(X), long(Y), long(Z), long(X),
+                 long(Y), long(Z), s->phys_final_time);
+    std::fclose(fp);
+    for (int k = 0; k < kCounters; ++k) {
+        const std::string name = "cnt_" + std::to_string(k);
+        fp = std::fopen(state_file(dir, name.c_str()).c_str(), "wt");
+        if (!fp) return fail(LBM355_EINVAL, "cannot create " + name);
+        std::fprintf(fp, "%d\n%le\n", s->counter_count[k], s->counter_period[k]);
+        std::fclose(fp);
+    }
+    // not in the reference layout: what an exact restart of this library needs
+    fp = std::fopen(state_file(dir, "lbm355").c_str(), "wt");
+    if (!fp) return fail(LBM355_EINVAL, "cannot create lbm355 state");
+    std::fprintf(fp, "%d\n%d\n%ld\n%ld\n", c->storage, c->model, long(s->stat_prev_iter), long(c->stats_steps));
+    std::fclose(fp);
+    return c->storage == LBM355_F64 ? save_arrays<double>(c, dir, pops) : save_arrays<float>(c, dir, pops);
+}
+
+int lbm355_case_load_state(lbm355_case* c, const char* dir, lbm355_state_desc* s)
+{
+    if (!c || !dir || !s) return fail(LBM355_EINVAL, "null argument");
+    FILE* fp = std::fopen(state_file(dir, "config").c_str(), "rt");
+    if (!fp) return fail(LBM355_EINVAL, std::string("no saved state in ") + dir);
+    int it = 0;
+    long gx = 0, gy = 0, gz = 0, lx = 0, ly = 0, lz = 0;
+    double ft = 0.0;
+    const int nr = std::fscanf(fp, "%d %ld %ld %ld %ld %ld %ld %le", &it, &gx, &gy, &gz, &lx, &ly, &lz, &ft);
+    std::fclose(fp);
+    if (nr != 8) return fail(LBM355_EINVAL, "malformed config");
+    if (gx != c->d.nx || gy != c->d.ny || gz != c->d.nz) return fail(LBM355_EINVAL, "saved state has other dimensions");
+    s->phys_final_time = ft;
+    s->iterations = it;
+    for (int k = 0; k < kCounters; ++k) {
+        const std::string name = "cnt_" + std::to_string(k);
+        fp = std::fopen(state_file(dir, name.c_str()).c_str(), "rt");
+        if (!fp) return fail(LBM355_EINVAL, "missing " + name);
+        int cnt = 0;
+        double per = -1.0;
+        const int r = std::fscanf(fp, "%d %le", &cnt, &per);
+        std::fclose(fp);
+        if (r != 2) return fail(LBM355_EINVAL, "malformed " + name);
+        s->counter_count[k] = cnt;
+        s->counter_period[k] = per;
+    }
+    long prev = 0, samples = -1;
+    int st = -1, mo = -1;
+    if ((fp = std::fopen(state_file(dir, "lbm355").c_str(), "rt"))) {
+        if (std::fscanf(fp, "%d %d %ld %ld", &st, &mo, &prev, &samples) != 4) samples = -1;
+        std::fclose(fp);
+    }
+    s->stat_prev_iter = prev;
+    std::vector<double> pops;
+    std::vector<std::vector<double>> fields;
+    int rc;
+    // a reference state is dreal = float unless this case stores FP64
+    const bool f64 = c->storage == LBM355_F64;
+    rc = f64 ? load_arrays<double>(c, dir, it, pops, fields) : load_arrays<float>(c, dir, it, pops, fields);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->steps = it;
+    if ((rc = set_populations(c, pops))) return rc;
+    // macro rho,u of the saved step, statistics sums, body force, strain
+    for (int id = 0; id < kMacroTcpc; ++id) {
+        const int f = kMacroMap[id];
+        if (f < 0) continue;
+        if (f <= LBM355_FIELD_VZ)
+            rc = f64 ? scatter_dense<double>(*c, c->macro.p, 4, f, fields[id].data())
+                     : scatter_dense<float>(*c, c->macro.p, 4, f, fields[id].data());
+        else if (f >= LBM355_FIELD_SUM_RUX && f <= LBM355_FIELD_SUM_RRUZZ) {
+            if (c->d.stats_every > 0) rc = scatter_dense<double>(*c, c->stats.p, kStatFields, f - LBM355_FIELD_SUM_RUX,
+                                                                fields[id].data());
+        } else if (f >= LBM355_FIELD_FX && f <= LBM355_FIELD_FZ) {
+            if (c->force.p)
+                rc = f64 ? scatter_dense<double>(*c, c->force.p, 3, f - LBM355_FIELD_FX, fields[id].data())
+                         : scatter_dense<float>(*c, c->force.p, 3, f - LBM355_FIELD_FX, fields[id].data());
+        } else if (f >= LBM355_FIELD_S11 && f <= LBM355_FIELD_S33) {
+            rc = f64 ? scatter_dense<double>(*c, c->strain.p, 6, f - LBM355_FIELD_S11, fields[id].data())
+                     : scatter_dense<float>(*c, c->strain.p, 6, f - LBM355_FIELD_S11, fields[id].data());
+        }
+        if (rc) return rc;
+    }
+    // statistics: samples since the last reset; FP32 blocks restart here
+    const int64_t k = c->d.stats_every > 0 ? c->d.stats_every : 1;
+    c->stats_steps = samples >= 0 ? samples : int64_t(it) / k - int64_t(prev) / k;
+    c->stats_block0 = c->stats_steps;
+    c->stats_first = c->steps;
+    if (c->d.stats_every > 0 && (rc = set_case_u64(*c, &c->host.stats_epoch, uint64_t(c->steps)))) return rc;
+    c->macro_valid_step = c->steps - 1;
+    c->strain_valid = false;
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return LBM355_OK;
 }
 

@@ -561,12 +561,15 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
     const int parity = int(step & 1);
     // per-step output decisions, wave-uniform (32-bit step arithmetic)
     const uint32_t next = uint32_t(step) + 1u;
-    // statistics sample index since the last reset; samples are summed in
-    // FP32 blocks of kStatBlock and each block is added to the FP64 sums
-    const uint32_t snext = uint32_t(step - C.stats_epoch) + 1u;
-    const bool do_stats = C.stats && C.stats_every > 0 &&
-                          (C.stats_every == 1 || snext % uint32_t(C.stats_every) == 0u);
-    const uint32_t spos = (C.stats_every == 1 ? snext - 1u : snext / uint32_t(C.stats_every) - 1u) % kStatBlock;
+    // statistics are sampled at the steps whose index + 1 is a multiple of
+    // stats_every (as MacroTcpc every step when it is 1); samples are summed
+    // in FP32 blocks of kStatBlock counted from the epoch (the last reset or
+    // restart) and each block is added to the FP64 sums
+    const uint32_t kev = uint32_t(C.stats_every > 0 ? C.stats_every : 1);
+    const uint32_t s32 = uint32_t(step), e32 = uint32_t(C.stats_epoch);
+    const bool do_stats = C.stats && C.stats_every > 0 && (kev == 1u || (s32 + 1u) % kev == 0u);
+    const uint32_t nsamp = kev == 1u ? s32 - e32 + 1u : (s32 + 1u) / kev - e32 / kev;  // samples since the epoch
+    const uint32_t spos = (nsamp - 1u) % kStatBlock;
     const rsrc_t PR = make_rsrc(C.stats_part, C.stats_bytes / 2);
     const bool do_macro = C.macro && (step == C.macro_step ||
                                       (C.macro_every > 0 && next % uint32_t(C.macro_every) == 0u));

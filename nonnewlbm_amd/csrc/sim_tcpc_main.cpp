@@ -52,6 +52,8 @@ struct Options {
     double ivc_umax = -1.0, svc_umax = -1.0;
     double pulse_amp = 0.0, pulse_period = 0.0;
     int stats_every = 1;
+    double max_wall_time = -1.0;   // the reference's wallTime (state.h:256), seconds
+    bool save_at_end = false;
 };
 
 struct Material {
@@ -99,7 +101,12 @@ void usage()
               << "  --ivc-umax V --svc-umax V              parabolic peak velocities in m/s\n"
               << "                                         (default: twice the plug mean velocity)\n"
               << "  --pulse-amp A --pulse-period T         u(t) = u (1 + A sin(2 pi t / T)), T in s\n"
-              << "  --stats-every K                        statistics sum every K steps, default 1\n";
+              << "  --stats-every K                        statistics sum every K steps, default 1\n"
+              << "  --max-wall-time S                      save results_<id>/current_state and stop after S\n"
+              << "                                         seconds (creates the loadstate flag file)\n"
+              << "  --save-state-at-end                    save results_<id>/current_state when the run ends\n"
+              << "Restart: a file results_<id>/loadstate loads results_<id>/current_state at start;\n"
+              << "a file results_<id>/savestate saves it at the next print event.\n";
 }
 
 Options parse_args(int argc, char** argv)
@@ -133,6 +140,8 @@ Options parse_args(int argc, char** argv)
         else if (arg == "--pulse-amp") o.pulse_amp = std::atof(value("--pulse-amp"));
         else if (arg == "--pulse-period") o.pulse_period = std::atof(value("--pulse-period"));
         else if (arg == "--stats-every") o.stats_every = std::atoi(value("--stats-every"));
+        else if (arg == "--max-wall-time") o.max_wall_time = std::atof(value("--max-wall-time"));
+        else if (arg == "--save-state-at-end") o.save_at_end = true;
         else throw std::runtime_error("Unknown argument '" + arg + "'");
     }
     return o;
@@ -463,7 +472,44 @@ int run(const Options& opt)
     vtk_c.period = opt.vtk_period;
     reset_c.period = opt.stat_reset_period;
     int64_t it = 0, prev_iter = 0, info_iter = 0;
-    const double start_time = 0.0;
+    double start_time = 0.0;
+    const std::string state_dir = L.dir + "/current_state";
+    auto flag = [&](const char* name) { return L.dir + "/" + name; };
+    auto exists = [](const std::string& f) {
+        struct stat st;
+        return ::stat(f.c_str(), &st) == 0;
+    };
+    // counter slots of the reference (state.h:72): STAT_RESET 0, PRINT 2, VTK3D 5, SAVESTATE 9
+    auto save_state = [&](bool forced) {
+        lbm355_state_desc sd;
+        std::memset(&sd, 0, sizeof sd);
+        sd.phys_final_time = opt.steps > 0 ? double(opt.steps) * dt : opt.final_time;
+        for (int k = 0; k < 11; ++k) sd.counter_period[k] = -1.0;
+        sd.counter_count[0] = reset_c.count; sd.counter_period[0] = reset_c.period;
+        sd.counter_count[2] = print_c.count; sd.counter_period[2] = print_c.period;
+        sd.counter_count[5] = vtk_c.count; sd.counter_period[5] = vtk_c.period;
+        sd.counter_count[9] = 1;
+        sd.stat_prev_iter = prev_iter;
+        L.log("[saveState invoked]");
+        check(lbm355_case_save_state(c, state_dir.c_str(), &sd), "save state");
+        if (forced) {
+            if (FILE* f = std::fopen(flag("loadstate").c_str(), "wt")) std::fclose(f);
+        }
+    };
+    if (exists(flag("loadstate"))) {
+        // execute(): loadState instead of the initial state (core.h:1511-1517)
+        lbm355_state_desc sd;
+        std::memset(&sd, 0, sizeof sd);
+        L.log("[loadState invoked]");
+        check(lbm355_case_load_state(c, state_dir.c_str(), &sd), "load state");
+        it = sd.iterations;
+        info_iter = it;
+        prev_iter = sd.stat_prev_iter;
+        reset_c.count = sd.counter_count[0];
+        print_c.count = sd.counter_count[2];
+        vtk_c.count = sd.counter_count[5];
+        start_time = double(it) * dt;
+    }
     const auto wall0 = std::chrono::steady_clock::now();
     auto info_clock = wall0;
     const double sites = double(d.nx) * d.ny * d.nz;
@@ -492,6 +538,11 @@ int run(const Options& opt)
         if (print_c.action(t)) {
             write_info = true;
             print_c.count++;
+            if (exists(flag("savestate"))) {
+                save_state(false);
+                ::remove(flag("savestate").c_str());
+                if (FILE* f = std::fopen(flag("savestate_done").c_str(), "wt")) std::fclose(f);
+            }
         }
         if (reset_c.action(t)) {
             check(lbm355_case_reset_stats(c), "statistics reset");
@@ -537,8 +588,17 @@ int run(const Options& opt)
         check(lbm355_case_step(c, k - it), "step");
         it = k;
         after_update();
+        if (opt.max_wall_time > 0 &&
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count() >= opt.max_wall_time) {
+            // wallTimeReached: save the state, flag it for loading, stop (core.h:1580-1591)
+            check(lbm355_case_sync(c), "sync");
+            L.log("maximum wall time reached");
+            save_state(true);
+            break;
+        }
         if (finished(it)) {
             L.log(opt.steps > 0 ? "requested steps reached" : "physFinalTime reached");
+            if (opt.save_at_end) save_state(true);
             break;
         }
         if (terminate) {

@@ -120,6 +120,25 @@ class Case:
     def reset_stats(self) -> None:
         check(self.lib.lbm355_case_reset_stats(self.h), "reset_stats")
 
+    def save_state(self, directory: str | Path, final_time: float = 0.0, counters=None,
+                   stat_prev_iter: int = 0) -> None:
+        """Checkpoint in the reference's current_state layout; counters is a
+        list of 11 (count, period) pairs (STAT_RESET ... VTK3DCUT)."""
+        s = _abi.StateDesc()
+        s.phys_final_time = float(final_time)
+        for k in range(11):
+            cnt, per = (counters[k] if counters else (0, -1.0))
+            s.counter_count[k] = int(cnt)
+            s.counter_period[k] = float(per)
+        s.stat_prev_iter = int(stat_prev_iter)
+        check(self.lib.lbm355_case_save_state(self.h, str(directory).encode(), C.byref(s)), "save_state")
+
+    def load_state(self, directory: str | Path) -> dict:
+        s = _abi.StateDesc()
+        check(self.lib.lbm355_case_load_state(self.h, str(directory).encode(), C.byref(s)), "load_state")
+        return {"iterations": s.iterations, "final_time": s.phys_final_time, "stat_prev_iter": s.stat_prev_iter,
+                "counters": [(s.counter_count[k], s.counter_period[k]) for k in range(11)]}
+
     def write_vtk(self, path: str | Path, *, phys_density: float, phys_nu_inf: float, time: float, cycle: int,
                   stat_steps: int) -> None:
         o = OutputDesc()

@@ -75,3 +75,30 @@ def test_cli_zero_area_terminates(gpu_lib, tmp_path):
     assert "error: zero inlet area detected: left=0 right=0" in log
     assert "terminate flag triggered" in log
     assert (tmp_path / "results_z" / "vtk3D" / "rank000_data_0.vtk").exists()
+
+
+def test_cli_restart_through_loadstate(gpu_lib, tmp_path):
+    """--save-state-at-end writes results_<id>/current_state and the loadstate
+    flag; the next run with the same id continues from there (core.h:1511-1517)
+    and its output equals an uninterrupted run."""
+    g = tcpc_mask(0.25)
+    write_wall_vtk(tmp_path / "geometry.vtk", g)
+    u = lattice_units(material_by_name("newtonian"), float(np.float32(g.dl)), 1e-4)
+    common = ["--material", "newtonian", "--ivc-mls", "0.4", "--svc-mls", "0.2", "--lbm-viscosity", "1e-4",
+              "--vtk-period", str(20 * u.dt * 0.999999), "--print-period", "-1"]
+    r = _run(common + ["--case-id", "split", "--steps", "40", "--save-state-at-end"], tmp_path)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert (tmp_path / "results_split" / "loadstate").exists()
+    assert (tmp_path / "results_split" / "current_state" / "df_0_rank000").exists()
+    r = _run(common + ["--case-id", "split", "--steps", "60"], tmp_path)
+    assert r.returncode == 0, r.stderr + r.stdout
+    assert "[loadState invoked]" in (tmp_path / "results_split" / "log_rank000").read_text()
+    r = _run(common + ["--case-id", "whole", "--steps", "60"], tmp_path)
+    assert r.returncode == 0, r.stderr + r.stdout
+    names = sorted(p.name for p in (tmp_path / "results_split" / "vtk3D").glob("*.vtk"))
+    assert names == [f"rank000_data_{i}.vtk" for i in range(4)]
+    a = read_output_vtk(tmp_path / "results_split" / "vtk3D" / "rank000_data_3.vtk")
+    b = read_output_vtk(tmp_path / "results_whole" / "vtk3D" / "rank000_data_3.vtk")
+    for name in ("velocity", "lbm_rho", "mean_velocity"):
+        np.testing.assert_allclose(a[name], b[name], rtol=1e-5, atol=1e-12, err_msg=name)
+    np.testing.assert_array_equal(a["velocity"], b["velocity"])
