@@ -190,6 +190,12 @@ int lbm355_case_sync(lbm355_case* c);
    workgroup, 1, 2, 4 or 8 (default 4).  No counterpart in the reference, whose
    launch shape is the y-block size (--block-size, sim_tcpc.cu:36). */
 int lbm355_case_set_shape(lbm355_case* c, int bricks_per_group);
+/* cache policy of the even step's population reads and writes: 1 streams them
+   with the non-temporal hint, 0 uses the default policy, -1 (default) picks 1
+   when the case holds more than 448 MiB of device memory (larger than the
+   Infinity Cache keeps between steps).  Results do not depend on it.  No
+   counterpart in the reference. */
+int lbm355_case_set_nontemporal(lbm355_case* c, int mode);
 
 /* dense double copy of one field; bytes must be >= nx*ny*nz*8 */
 int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes);

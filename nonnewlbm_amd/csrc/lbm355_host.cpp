@@ -210,6 +210,7 @@ struct lbm355_case {
         origin, base, devcase, list, bc_list, flux;
     CaseDev host{};
     int bricks_per_group = 4;
+    int nontemporal = -1;               // even-step nt hint: -1 auto (by working set), 0 off, 1 on
 
     ~lbm355_case()
     {
@@ -587,6 +588,17 @@ int set_case_f64(lbm355_case& c, double* host_field, double v)
 
 StepLaunch single_launch(lbm355_case& c);
 
+// device memory held by a case
+int64_t case_device_bytes(const lbm355_case& c)
+{
+    int64_t bytes = 0;
+    for (const DevBuf* b : {&c.f, &c.stats, &c.stats_part, &c.macro, &c.strain, &c.force, &c.nbr, &c.label, &c.bc_base,
+                            &c.bc_cell_d, &c.bc_read, &c.bc_wmask, &c.bc_factor, &c.bc_label, &c.origin, &c.base,
+                            &c.devcase, &c.list, &c.bc_list, &c.flux})
+        bytes += int64_t(b->bytes);
+    return bytes;
+}
+
 // one time step of a case: the fd-force pre-kernels (velocity, strain, body
 // force) when that Carreau-Yasuda mode is on, then the fused step
 int launch_case_step(lbm355_case& c, const StepLaunch& L)
@@ -609,6 +621,7 @@ StepLaunch single_launch(lbm355_case& c)
     L.base = static_cast<const int64_t*>(c.base.p);
     L.rel = c.steps;
     L.bricks_per_group = c.bricks_per_group;
+    L.nontemporal = c.nontemporal >= 0 ? c.nontemporal : (case_device_bytes(c) > kNontemporalBytes ? 1 : 0);
     L.bc_list = static_cast<const uint2*>(c.bc_list.p);
     L.n_bc_list = c.n_bc;
     return L;
@@ -795,12 +808,7 @@ int lbm355_case_info_get(const lbm355_case* c, lbm355_case_info* o)
     o->active_cells = int64_t(c->d.nx) * c->d.ny * c->d.nz - c->lab.n_wall;
     o->bricks_active = c->n_active;
     o->bricks_stored = c->n_stored;
-    int64_t bytes = 0;
-    for (const DevBuf* b : {&c->f, &c->stats, &c->stats_part, &c->macro, &c->strain, &c->force, &c->nbr, &c->label, &c->bc_base,
-                            &c->bc_cell_d, &c->bc_read, &c->bc_wmask, &c->bc_factor, &c->bc_label, &c->origin,
-                            &c->base, &c->devcase, &c->list, &c->bc_list, &c->flux})
-        bytes += int64_t(b->bytes);
-    o->device_bytes = bytes;
+    o->device_bytes = case_device_bytes(*c);
     o->steps = c->steps;
     o->terminate = c->lab.terminate ? 1 : 0;
     return LBM355_OK;
@@ -843,6 +851,13 @@ int lbm355_case_set_shape(lbm355_case* c, int bpg)
 {
     if (!c || !(bpg == 1 || bpg == 2 || bpg == 4 || bpg == 8)) return fail(LBM355_EINVAL, "bricks per group");
     c->bricks_per_group = bpg;
+    return LBM355_OK;
+}
+
+int lbm355_case_set_nontemporal(lbm355_case* c, int mode)
+{
+    if (!c || mode < -1 || mode > 1) return fail(LBM355_EINVAL, "nontemporal mode is -1, 0 or 1");
+    c->nontemporal = mode;
     return LBM355_OK;
 }
 
@@ -1546,6 +1561,9 @@ StepLaunch batch_launch(lbm355_batch& b)
     L.base = static_cast<const int64_t*>(b.base.p);
     L.rel = b.launches - b.base_host;
     L.bricks_per_group = b.bricks_per_group;
+    int64_t bytes = 0;
+    for (const lbm355_case* c : b.cases) bytes += case_device_bytes(*c);
+    L.nontemporal = bytes > kNontemporalBytes ? 1 : 0;
     L.bc_list = static_cast<const uint2*>(b.bc_list.p);
     L.n_bc_list = b.n_bc;
     return L;

@@ -83,6 +83,11 @@ struct BrickRef {
     uint32_t brick;
 };
 
+// device working set above which the even step streams its populations with
+// the non-temporal hint (measured crossover on MI355X between 331 MB, where the
+// Infinity Cache still serves the next step, and 509 MB)
+constexpr int64_t kNontemporalBytes = int64_t(448) << 20;
+
 // launch descriptor of one fused step over a brick list
 struct StepLaunch {
     const BrickRef* list;
@@ -92,6 +97,7 @@ struct StepLaunch {
     const int64_t* base;      // device launch counter; this launch is *base + rel
     int64_t rel;
     int32_t bricks_per_group; // 1, 2, 4, 8 wavefronts per workgroup
+    int32_t nontemporal;      // 1: even-step population accesses with the nt hint
     // BC pre-pass list
     const uint2* bc_list;     // (case_id, bc slot)
     int32_t n_bc_list;

@@ -82,6 +82,10 @@ class Case:
         """4x4x4 bricks per workgroup of the fused step (1, 2, 4 or 8)."""
         check(self.lib.lbm355_case_set_shape(self.h, int(bricks_per_group)), "case_set_shape")
 
+    def set_nontemporal(self, mode: int) -> None:
+        """-1 auto (by working set), 0 default cache policy, 1 non-temporal even step."""
+        check(self.lib.lbm355_case_set_nontemporal(self.h, int(mode)), "case_set_nontemporal")
+
     def time_kernel(self, n: int) -> tuple[float, float]:
         """Run n steps; returns (ms per whole step, ms per fused-kernel launch)."""
         step_ms, kern_ms = C.c_double(), C.c_double()

@@ -228,3 +228,24 @@ def test_output_strain_matches_oracle(gpu_lib):
     for name in ("s11", "s12", "s13", "s22", "s32", "s33"):
         np.testing.assert_allclose(gpu.field(name), cpu.field(name), rtol=0, atol=1e-13, err_msg=name)
     assert np.abs(cpu.field("s12")[fl]).max() > 1e-6
+
+
+@pytest.mark.parametrize("storage", [_abi.F32, _abi.F16_OFFSET])
+def test_nontemporal_policy_changes_nothing(gpu_lib, storage):
+    """The junction x2 (above the working set where the even step streams with
+    the non-temporal hint) gives the same bits with the hint forced on and off."""
+    from nonnewlbm_amd.geometry import tcpc_mask
+    g = tcpc_mask(2.0)
+    kw = dict(storage=storage, collision=_abi.CUM, nu_ref=0.01, profile=_abi.PROFILE_PARABOLIC, ivc_umax=0.04,
+              svc_umax=0.03, ivc_center=g.meta["ivc"]["center"], svc_center=g.meta["svc"]["center"],
+              ivc_radius=g.meta["ivc"]["radius"], svc_radius=g.meta["svc"]["radius"], dl=g.dl, dt=1e-5)
+    a = Case(g.flat(), g.dims, **kw)
+    a.set_nontemporal(1)
+    a.step(6)
+    va = a.field("vx")
+    a.close()
+    b = Case(g.flat(), g.dims, **kw)
+    b.set_nontemporal(0)
+    b.step(6)
+    np.testing.assert_array_equal(va, b.field("vx"))
+    assert np.abs(va).max() > 0
