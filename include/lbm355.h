@@ -268,10 +268,13 @@ int lbm355_batch_step(lbm355_batch* b, int64_t n);
 int lbm355_batch_destroy(lbm355_batch* b);
 /* launch shape: 4x4x4 bricks (wavefronts) per workgroup, 1, 2, 4 or 8 */
 int lbm355_batch_set_shape(lbm355_batch* b, int bricks_per_group);
-/* times every launch shape on 10 real steps of the batch each (2 warm-up + 8
-   timed; the steps count as simulation steps of every member) and keeps the
-   fastest */
-int lbm355_batch_autotune(lbm355_batch* b, int* best_bricks_per_group);
+/* times every launch shape (1, 2, 4, 8 bricks per workgroup) with and
+   without the non-temporal even step on 10 real steps each (2 warm-up + 8
+   timed; they count as simulation steps of every member) and keeps the
+   fastest; *best = bricks_per_group | nontemporal << 8 */
+int lbm355_batch_autotune(lbm355_batch* b, int* best);
+/* the same for one case (80 real steps) */
+int lbm355_case_autotune(lbm355_case* c, int* best);
 
 /* --- serving pool ----------------------------------------------------------
  * No counterpart in the reference, which runs one case per process and queues

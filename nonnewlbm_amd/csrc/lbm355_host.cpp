@@ -1532,6 +1532,7 @@ struct lbm355_batch {
     int storage = 1, model = 0;
     int use_graph = 0;
     int bricks_per_group = 4;
+    int nontemporal = -1;        // -1 auto (by the members' working set), 0 off, 1 on
     hipStream_t stream = nullptr;
     DevBuf devcases, list, bc_list, base;
     int n_list = 0, n_bc = 0;
@@ -1563,7 +1564,7 @@ StepLaunch batch_launch(lbm355_batch& b)
     L.bricks_per_group = b.bricks_per_group;
     int64_t bytes = 0;
     for (const lbm355_case* c : b.cases) bytes += case_device_bytes(*c);
-    L.nontemporal = bytes > kNontemporalBytes ? 1 : 0;
+    L.nontemporal = b.nontemporal >= 0 ? b.nontemporal : (bytes > kNontemporalBytes ? 1 : 0);
     L.bc_list = static_cast<const uint2*>(b.bc_list.p);
     L.n_bc_list = b.n_bc;
     return L;
@@ -1736,34 +1737,63 @@ int lbm355_batch_step(lbm355_batch* b, int64_t n)
 int lbm355_batch_autotune(lbm355_batch* b, int* best)
 {
     if (!b) return fail(LBM355_EINVAL, "null batch");
-    // real steps of the batch, an even number per shape so both AA parities
-    // count; the fastest shape is kept
+    // real steps of the batch, an even number per configuration so both AA
+    // parities count; the fastest launch shape and cache policy are kept
     const int shapes[4] = {1, 2, 4, 8};
     const int steps = 8;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     float best_ms = 0.f;
-    int best_bpg = b->bricks_per_group;
+    int best_bpg = b->bricks_per_group, best_nt = b->nontemporal;
     int rc = LBM355_OK;
-    for (int k = 0; k < 4 && rc == LBM355_OK; ++k) {
-        b->bricks_per_group = shapes[k];
-        drop_graph(*b);
-        if ((rc = batch_enqueue(b, 2)) || (rc = batch_finish(b))) break;  // warm-up (and graph capture)
-        if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
-        if ((rc = batch_enqueue(b, steps))) break;
-        if (hipEventRecord(e1, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
-        if ((rc = batch_finish(b))) break;
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (k == 0 || ms < best_ms) { best_ms = ms; best_bpg = shapes[k]; }
-    }
+    bool first = true;
+    for (int nt = 0; nt <= 1 && rc == LBM355_OK; ++nt)
+        for (int k = 0; k < 4 && rc == LBM355_OK; ++k) {
+            b->bricks_per_group = shapes[k];
+            b->nontemporal = nt;
+            drop_graph(*b);
+            if ((rc = batch_enqueue(b, 2)) || (rc = batch_finish(b))) break;  // warm-up (and graph capture)
+            if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
+            if ((rc = batch_enqueue(b, steps))) break;
+            if (hipEventRecord(e1, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
+            if ((rc = batch_finish(b))) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (first || ms < best_ms) { best_ms = ms; best_bpg = shapes[k]; best_nt = nt; }
+            first = false;
+        }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     b->bricks_per_group = best_bpg;
+    b->nontemporal = best_nt;
     drop_graph(*b);
-    if (best) *best = best_bpg;
+    if (best) *best = best_bpg | (best_nt << 8);
+    return LBM355_OK;
+}
+
+int lbm355_case_autotune(lbm355_case* c, int* best)
+{
+    if (!c) return fail(LBM355_EINVAL, "null case");
+    const int shapes[4] = {1, 2, 4, 8};
+    double best_ms = 0.0;
+    int best_bpg = c->bricks_per_group, best_nt = c->nontemporal;
+    bool first = true;
+    int rc;
+    for (int nt = 0; nt <= 1; ++nt)
+        for (int k = 0; k < 4; ++k) {
+            c->bricks_per_group = shapes[k];
+            c->nontemporal = nt;
+            if ((rc = lbm355_case_step(c, 2))) return rc;  // warm-up
+            double step_ms = 0.0;
+            if ((rc = lbm355_case_time_kernel(c, 8, &step_ms, nullptr))) return rc;
+            if (first || step_ms < best_ms) { best_ms = step_ms; best_bpg = shapes[k]; best_nt = nt; }
+            first = false;
+        }
+    c->bricks_per_group = best_bpg;
+    c->nontemporal = best_nt;
+    if (best) *best = best_bpg | (best_nt << 8);
     return LBM355_OK;
 }
 

@@ -86,6 +86,13 @@ class Case:
         """-1 auto (by working set), 0 default cache policy, 1 non-temporal even step."""
         check(self.lib.lbm355_case_set_nontemporal(self.h, int(mode)), "case_set_nontemporal")
 
+    def autotune(self) -> tuple[int, int]:
+        """Time launch shapes x cache policy on 80 real steps; keeps and
+        returns the fastest (bricks_per_group, nontemporal)."""
+        best = C.c_int()
+        check(self.lib.lbm355_case_autotune(self.h, C.byref(best)), "case_autotune")
+        return best.value & 0xFF, best.value >> 8
+
     def time_kernel(self, n: int) -> tuple[float, float]:
         """Run n steps; returns (ms per whole step, ms per fused-kernel launch)."""
         step_ms, kern_ms = C.c_double(), C.c_double()
@@ -171,10 +178,11 @@ class Batch:
     def set_shape(self, bricks_per_group: int) -> None:
         check(self.lib.lbm355_batch_set_shape(self.h, int(bricks_per_group)), "batch_set_shape")
 
-    def autotune(self) -> int:
+    def autotune(self) -> tuple[int, int]:
+        """Returns the fastest (bricks_per_group, nontemporal), now in use."""
         best = C.c_int()
         check(self.lib.lbm355_batch_autotune(self.h, C.byref(best)), "batch_autotune")
-        return best.value
+        return best.value & 0xFF, best.value >> 8
 
     def close(self) -> None:
         if getattr(self, "h", None):
