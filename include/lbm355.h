@@ -227,7 +227,11 @@ typedef struct lbm355_output_desc {
     double time;           /* physical time written to FIELD TIME                 */
     int32_t cycle;         /* VTK counter                                         */
     int32_t stat_steps;    /* iterations - prev_iter (sim_tcpc.cu:720)            */
-    int32_t reserved[6];
+    int32_t material_cy;   /* 1: the material is Carreau-Yasuda, so nu/mu/wss use
+                              the case's cy_* parameters even when the coupling
+                              is off (localNuLbm reads the material,
+                              sim_tcpc.cu:697-707); 0: follow cy_enabled        */
+    int32_t reserved[5];
 } lbm355_output_desc;
 int lbm355_case_write_vtk(lbm355_case* c, const char* path, const lbm355_output_desc* o);
 

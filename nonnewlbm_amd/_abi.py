@@ -95,7 +95,7 @@ class StateDesc(C.Structure):
 class OutputDesc(C.Structure):
     _fields_ = [
         ("phys_density", C.c_double), ("phys_nu_inf", C.c_double), ("time", C.c_double),
-        ("cycle", C.c_int32), ("stat_steps", C.c_int32), ("reserved", C.c_int32 * 6),
+        ("cycle", C.c_int32), ("stat_steps", C.c_int32), ("material_cy", C.c_int32), ("reserved", C.c_int32 * 5),
     ]
 
 

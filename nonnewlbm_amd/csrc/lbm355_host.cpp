@@ -1434,7 +1434,7 @@ int lbm355_case_write_vtk(lbm355_case* c, const char* path, const lbm355_output_
     const double vconv = dl / dt;
     const double diff = std::max(1.0, double(o->stat_steps));
     const double cs2 = vconv * vconv / 3.0;
-    const bool cy = c->d.cy_enabled != 0;
+    const bool cy = c->d.cy_enabled != 0 || o->material_cy != 0;
     auto wall_at = [&](int x, int y, int z) -> bool {
         if (x < 0 || y < 0 || z < 0 || x >= X || y >= Y || z >= Z) return false;
         return c->lab.wall[c->cell_index(x, y, z)] != 0;

@@ -528,6 +528,7 @@ int run(const Options& opt)
             std::memset(&o, 0, sizeof o);
             o.phys_density = mat.density;
             o.phys_nu_inf = mat.nu_inf;
+            o.material_cy = mat.carreau_yasuda ? 1 : 0;
             o.time = t;
             o.cycle = vtk_c.count;
             o.stat_steps = int32_t(it - prev_iter);

@@ -151,13 +151,14 @@ class Case:
                 "counters": [(s.counter_count[k], s.counter_period[k]) for k in range(11)]}
 
     def write_vtk(self, path: str | Path, *, phys_density: float, phys_nu_inf: float, time: float, cycle: int,
-                  stat_steps: int) -> None:
+                  stat_steps: int, material_cy: bool = False) -> None:
         o = OutputDesc()
         o.phys_density = phys_density
         o.phys_nu_inf = phys_nu_inf
         o.time = time
         o.cycle = int(cycle)
         o.stat_steps = int(stat_steps)
+        o.material_cy = 1 if material_cy else 0
         check(self.lib.lbm355_case_write_vtk(self.h, str(path).encode(), C.byref(o)), "write_vtk")
 
 
