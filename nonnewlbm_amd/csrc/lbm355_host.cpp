@@ -207,7 +207,7 @@ struct lbm355_case {
 
     hipStream_t stream = nullptr;
     DevBuf f, stats, stats_part, macro, strain, force, nbr, label, bc_base, bc_cell_d, bc_read, bc_wmask, bc_factor, bc_label,
-        origin, base, devcase, list, bc_list, flux;
+        origin, base, devcase, list, flux;
     CaseDev host{};
     int bricks_per_group = 4;
     int nontemporal = -1;               // even-step nt hint: -1 auto (by working set), 0 off, 1 on
@@ -519,9 +519,6 @@ int setup_device(lbm355_case& c)
     std::vector<BrickRef> list(size_t(std::max(na, 1)));
     for (int b = 0; b < na; ++b) list[b] = BrickRef{0u, uint32_t(b)};
     if ((rc = upload(c.list, list))) return rc;
-    std::vector<uint2> bl(size_t(std::max(c.n_bc, 1)));
-    for (int s = 0; s < c.n_bc; ++s) bl[s] = make_uint2(0u, uint32_t(s));
-    if ((rc = upload(c.bc_list, bl))) return rc;
 
     CaseDev& h = c.host;
     std::memset(&h, 0, sizeof(h));
@@ -594,7 +591,7 @@ int64_t case_device_bytes(const lbm355_case& c)
     int64_t bytes = 0;
     for (const DevBuf* b : {&c.f, &c.stats, &c.stats_part, &c.macro, &c.strain, &c.force, &c.nbr, &c.label, &c.bc_base,
                             &c.bc_cell_d, &c.bc_read, &c.bc_wmask, &c.bc_factor, &c.bc_label, &c.origin, &c.base,
-                            &c.devcase, &c.list, &c.bc_list, &c.flux})
+                            &c.devcase, &c.list, &c.flux})
         bytes += int64_t(b->bytes);
     return bytes;
 }
@@ -608,7 +605,7 @@ int launch_case_step(lbm355_case& c, const StepLaunch& L)
                                               c.stream);
         if (r) return r;
     }
-    return launch_step(c.storage, c.model, 0, L, c.stream);
+    return launch_step(c.storage, c.model, L, c.stream);
 }
 
 StepLaunch single_launch(lbm355_case& c)
@@ -622,8 +619,6 @@ StepLaunch single_launch(lbm355_case& c)
     L.rel = c.steps;
     L.bricks_per_group = c.bricks_per_group;
     L.nontemporal = c.nontemporal >= 0 ? c.nontemporal : (case_device_bytes(c) > kNontemporalBytes ? 1 : 0);
-    L.bc_list = static_cast<const uint2*>(c.bc_list.p);
-    L.n_bc_list = c.n_bc;
     return L;
 }
 
@@ -1534,8 +1529,8 @@ struct lbm355_batch {
     int bricks_per_group = 4;
     int nontemporal = -1;        // -1 auto (by the members' working set), 0 off, 1 on
     hipStream_t stream = nullptr;
-    DevBuf devcases, list, bc_list, base;
-    int n_list = 0, n_bc = 0;
+    DevBuf devcases, list, base;
+    int n_list = 0;
     int64_t launches = 0;     // steps done by the batch
     int64_t base_host = 0;    // mirror of the device launch base
     std::vector<int64_t> start;  // per case: launch index of its step 0
@@ -1565,8 +1560,6 @@ StepLaunch batch_launch(lbm355_batch& b)
     int64_t bytes = 0;
     for (const lbm355_case* c : b.cases) bytes += case_device_bytes(*c);
     L.nontemporal = b.nontemporal >= 0 ? b.nontemporal : (bytes > kNontemporalBytes ? 1 : 0);
-    L.bc_list = static_cast<const uint2*>(b.bc_list.p);
-    L.n_bc_list = b.n_bc;
     return L;
 }
 
@@ -1590,7 +1583,6 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
     b->use_graph = graph;
     std::vector<CaseDev> cd;
     std::vector<BrickRef> list;
-    std::vector<uint2> bl;
     for (int i = 0; i < n; ++i) {
         lbm355_case* c = cases[i];
         if (!c) return fail(LBM355_EINVAL, "null case in batch");
@@ -1605,20 +1597,14 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
         b->start.push_back(-c->steps);
         b->epoch.push_back(c->stats_first);
         for (int k = 0; k < c->n_active; ++k) list.push_back(BrickRef{uint32_t(i), uint32_t(k)});
-        // each wavefront of the BC pre-pass serves one case (its buffer
-        // descriptors are wave-uniform): pad every case's run to 64 entries
-        for (int s = 0; s < c->n_bc; ++s) bl.push_back(make_uint2(uint32_t(i), uint32_t(s)));
-        while (bl.size() % kLanes) bl.push_back(make_uint2(uint32_t(i), 0xFFFFFFFFu));
     }
-    // interleave the cases' Morton-ordered brick runs by chunks so that the
-    // XCD-contiguous split of the list gives every XCD part of every case
+    // the cases' Morton-ordered brick runs follow each other; the
+    // XCD-contiguous split of the list gives each XCD consecutive bricks
     b->n_list = int(list.size());
-    b->n_bc = int(bl.size());
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     int rc;
     if ((rc = upload(b->devcases, cd))) return rc;
     if ((rc = upload(b->list, list))) return rc;
-    if ((rc = upload(b->bc_list, bl))) return rc;
     if (b->base.alloc(16) != 0) return fail(LBM355_ENOMEM, "launch counter");
     LAUNCH_TRY(launch_zero(b->base.p, 16, b->stream));
     // macro output only on explicit request inside a batch
@@ -1674,7 +1660,7 @@ int batch_enqueue(lbm355_batch* b, int64_t n)
             int r = 0;
             for (int s = 0; s < chunk && r == 0; ++s) {
                 L.rel = s;
-                r = launch_step(b->storage, b->model, 0, L, b->stream);
+                r = launch_step(b->storage, b->model, L, b->stream);
             }
             if (r == 0) r = launch_advance(static_cast<int64_t*>(b->base.p), chunk, b->stream);
             hipGraph_t g = nullptr;
@@ -1701,7 +1687,7 @@ int batch_enqueue(lbm355_batch* b, int64_t n)
     }
     for (; done < n; ++done) {
         const StepLaunch L = batch_launch(*b);
-        LAUNCH_TRY(launch_step(b->storage, b->model, 0, L, b->stream));
+        LAUNCH_TRY(launch_step(b->storage, b->model, L, b->stream));
         b->launches += 1;
     }
     for (lbm355_case* c : b->cases) {

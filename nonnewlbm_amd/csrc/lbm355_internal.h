@@ -98,9 +98,6 @@ struct StepLaunch {
     int64_t rel;
     int32_t bricks_per_group; // 1, 2, 4, 8 wavefronts per workgroup
     int32_t nontemporal;      // 1: even-step population accesses with the nt hint
-    // BC pre-pass list
-    const uint2* bc_list;     // (case_id, bc slot)
-    int32_t n_bc_list;
 };
 
 // storage kinds (same values as LBM355_F64 ...)
@@ -120,8 +117,8 @@ __host__ __device__ constexpr double lattice_weight(int q)
 }
 
 // launchers (lbm355_kernels.hip); all asynchronous on `stream`
-// phase 0: BC pre-pass + fused kernel, 1: pre-pass only, 2: fused kernel only
-int launch_step(int storage, int model, int phase, const StepLaunch& L, void* stream);
+// one fused step over the launch's brick list
+int launch_step(int storage, int model, const StepLaunch& L, void* stream);
 int launch_copy(void* dst, const void* src, int64_t bytes, void* stream);
 int launch_advance(int64_t* base, int64_t k, void* stream);
 int launch_fill_populations(int storage, void* f, int64_t n_bricks, void* stream);

@@ -1033,9 +1033,8 @@ static int check_launch()
 }
 
 template <int S, int MODEL>
-static int launch_step_t(const StepLaunch& L, int phase, hipStream_t st)
+static int launch_step_t(const StepLaunch& L, hipStream_t st)
 {
-    if (phase == 1) return check_launch();  // the BC reads are part of the fused kernel
     const int bpg = L.bricks_per_group;
     int groups = (L.n_list + bpg - 1) / bpg;
     groups = ((groups + 7) / 8) * 8;
@@ -1046,25 +1045,25 @@ static int launch_step_t(const StepLaunch& L, int phase, hipStream_t st)
 }
 
 template <int S>
-static int launch_step_s(int model, const StepLaunch& L, int phase, hipStream_t st)
+static int launch_step_s(int model, const StepLaunch& L, hipStream_t st)
 {
     switch (model) {
-    case M_CUM: return launch_step_t<S, M_CUM>(L, phase, st);
-    case M_BGK: return launch_step_t<S, M_BGK>(L, phase, st);
-    case M_MRT: return launch_step_t<S, M_MRT>(L, phase, st);
-    case M_KBC: return launch_step_t<S, M_KBC>(L, phase, st);
+    case M_CUM: return launch_step_t<S, M_CUM>(L, st);
+    case M_BGK: return launch_step_t<S, M_BGK>(L, st);
+    case M_MRT: return launch_step_t<S, M_MRT>(L, st);
+    case M_KBC: return launch_step_t<S, M_KBC>(L, st);
     default: g_kerr = "unknown collision"; return -1;
     }
 }
 
-int launch_step(int storage, int model, int phase, const StepLaunch& L, void* stream)
+int launch_step(int storage, int model, const StepLaunch& L, void* stream)
 {
     hipStream_t st = static_cast<hipStream_t>(stream);
     switch (storage) {
-    case S_F64: return launch_step_s<S_F64>(model, L, phase, st);
-    case S_F32: return launch_step_s<S_F32>(model, L, phase, st);
-    case S_F16: return launch_step_s<S_F16>(model, L, phase, st);
-    case S_BF16: return launch_step_s<S_BF16>(model, L, phase, st);
+    case S_F64: return launch_step_s<S_F64>(model, L, st);
+    case S_F32: return launch_step_s<S_F32>(model, L, st);
+    case S_F16: return launch_step_s<S_F16>(model, L, st);
+    case S_BF16: return launch_step_s<S_BF16>(model, L, st);
     default: g_kerr = "unknown storage"; return -1;
     }
 }
