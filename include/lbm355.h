@@ -196,6 +196,12 @@ int lbm355_case_set_shape(lbm355_case* c, int bricks_per_group);
    Infinity Cache keeps between steps).  Results do not depend on it.  No
    counterpart in the reference. */
 int lbm355_case_set_nontemporal(lbm355_case* c, int mode);
+/* at most waves_per_cu wavefronts of the fused step per CU (4..32), through an
+   LDS allocation the kernel does not use; 0 (default) leaves occupancy to the
+   register count.  Fewer resident wavefronts keep the odd step's neighbour
+   lines in L2 longer on cases larger than the Infinity Cache.  Results do not
+   depend on it.  No counterpart in the reference. */
+int lbm355_case_set_occupancy(lbm355_case* c, int waves_per_cu);
 
 /* dense double copy of one field; bytes must be >= nx*ny*nz*8 */
 int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes);
@@ -272,12 +278,14 @@ int lbm355_batch_step(lbm355_batch* b, int64_t n);
 int lbm355_batch_destroy(lbm355_batch* b);
 /* launch shape: 4x4x4 bricks (wavefronts) per workgroup, 1, 2, 4 or 8 */
 int lbm355_batch_set_shape(lbm355_batch* b, int bricks_per_group);
-/* times every launch shape (1, 2, 4, 8 bricks per workgroup) with and
-   without the non-temporal even step on 10 real steps each (2 warm-up + 8
-   timed; they count as simulation steps of every member) and keeps the
-   fastest; *best = bricks_per_group | nontemporal << 8 */
+int lbm355_batch_set_occupancy(lbm355_batch* b, int waves_per_cu);
+/* times every launch shape (1, 2, 4, 8 bricks per workgroup), with and
+   without the non-temporal even step, and with no cap, 16 or 12 wavefronts per
+   CU, on 10 real steps each (2 warm-up + 8 timed; they count as simulation
+   steps of every member) and keeps the fastest;
+   *best = bricks_per_group | nontemporal << 8 | waves_per_cu << 16 */
 int lbm355_batch_autotune(lbm355_batch* b, int* best);
-/* the same for one case (80 real steps) */
+/* the same for one case (240 real steps) */
 int lbm355_case_autotune(lbm355_case* c, int* best);
 
 /* --- serving pool ----------------------------------------------------------

@@ -211,6 +211,7 @@ struct lbm355_case {
     CaseDev host{};
     int bricks_per_group = 4;
     int nontemporal = -1;               // even-step nt hint: -1 auto (by working set), 0 off, 1 on
+    int waves_cap = 0;                  // wavefronts per CU of the fused step, 0 = no cap
 
     ~lbm355_case()
     {
@@ -608,6 +609,26 @@ int launch_case_step(lbm355_case& c, const StepLaunch& L)
     return launch_step(c.storage, c.model, L, c.stream);
 }
 
+// dynamic LDS per workgroup that leaves room for at most cap / bpg
+// workgroups (cap wavefronts) per CU; 0 for no cap.  The fused step uses no
+// LDS itself, the allocation only limits how many of its wavefronts share a
+// CU and hence the L2 footprint of the odd step's neighbour accesses.
+int occupancy_lds(int cap, int bpg)
+{
+    if (cap <= 0) return 0;
+    static int per_cu = 0, per_block = 0;
+    if (per_cu == 0) {
+        int dev = 0, v = 0;
+        (void)hipGetDevice(&dev);
+        per_cu = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess && v > 0
+                     ? v : 160 * 1024;
+        per_block = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0
+                        ? v : 64 * 1024;
+    }
+    const int groups = std::max(1, cap / bpg);
+    return std::min(per_cu / (groups + 1) + 1, per_block);
+}
+
 StepLaunch single_launch(lbm355_case& c)
 {
     StepLaunch L{};
@@ -619,6 +640,7 @@ StepLaunch single_launch(lbm355_case& c)
     L.rel = c.steps;
     L.bricks_per_group = c.bricks_per_group;
     L.nontemporal = c.nontemporal >= 0 ? c.nontemporal : (case_device_bytes(c) > kNontemporalBytes ? 1 : 0);
+    L.lds_bytes = occupancy_lds(c.waves_cap, c.bricks_per_group);
     return L;
 }
 
@@ -853,6 +875,14 @@ int lbm355_case_set_nontemporal(lbm355_case* c, int mode)
 {
     if (!c || mode < -1 || mode > 1) return fail(LBM355_EINVAL, "nontemporal mode is -1, 0 or 1");
     c->nontemporal = mode;
+    return LBM355_OK;
+}
+
+int lbm355_case_set_occupancy(lbm355_case* c, int waves_per_cu)
+{
+    if (!c || !(waves_per_cu == 0 || (waves_per_cu >= 4 && waves_per_cu <= 32)))
+        return fail(LBM355_EINVAL, "wavefronts per CU is 0 (no cap) or 4..32");
+    c->waves_cap = waves_per_cu;
     return LBM355_OK;
 }
 
@@ -1528,6 +1558,7 @@ struct lbm355_batch {
     int use_graph = 0;
     int bricks_per_group = 4;
     int nontemporal = -1;        // -1 auto (by the members' working set), 0 off, 1 on
+    int waves_cap = 0;           // wavefronts per CU of the fused step, 0 = no cap
     hipStream_t stream = nullptr;
     DevBuf devcases, list, base;
     int n_list = 0;
@@ -1547,6 +1578,18 @@ struct lbm355_batch {
 };
 
 namespace {
+// the autotuner's search space: bricks per workgroup x even-step cache policy
+// x wavefronts per CU (no cap, 16, 12)
+struct TuneConfig {
+    int bpg, nt, cap;
+};
+constexpr int kTuneConfigs = 4 * 2 * 3;
+TuneConfig tune_config(int k)
+{
+    const int shapes[4] = {1, 2, 4, 8}, caps[3] = {0, 16, 12};
+    return TuneConfig{shapes[k % 4], (k / 4) % 2, caps[k / 8]};
+}
+
 StepLaunch batch_launch(lbm355_batch& b)
 {
     StepLaunch L{};
@@ -1560,6 +1603,7 @@ StepLaunch batch_launch(lbm355_batch& b)
     int64_t bytes = 0;
     for (const lbm355_case* c : b.cases) bytes += case_device_bytes(*c);
     L.nontemporal = b.nontemporal >= 0 ? b.nontemporal : (bytes > kNontemporalBytes ? 1 : 0);
+    L.lds_bytes = occupancy_lds(b.waves_cap, b.bricks_per_group);
     return L;
 }
 
@@ -1621,6 +1665,15 @@ int lbm355_batch_set_shape(lbm355_batch* b, int bpg)
 {
     if (!b || !(bpg == 1 || bpg == 2 || bpg == 4 || bpg == 8)) return fail(LBM355_EINVAL, "bricks per group");
     b->bricks_per_group = bpg;
+    drop_graph(*b);
+    return LBM355_OK;
+}
+
+int lbm355_batch_set_occupancy(lbm355_batch* b, int waves_per_cu)
+{
+    if (!b || !(waves_per_cu == 0 || (waves_per_cu >= 4 && waves_per_cu <= 32)))
+        return fail(LBM355_EINVAL, "wavefronts per CU is 0 (no cap) or 4..32");
+    b->waves_cap = waves_per_cu;
     drop_graph(*b);
     return LBM355_OK;
 }
@@ -1724,62 +1777,65 @@ int lbm355_batch_autotune(lbm355_batch* b, int* best)
 {
     if (!b) return fail(LBM355_EINVAL, "null batch");
     // real steps of the batch, an even number per configuration so both AA
-    // parities count; the fastest launch shape and cache policy are kept
-    const int shapes[4] = {1, 2, 4, 8};
+    // parities count; the fastest launch shape, cache policy and wavefront
+    // cap are kept
     const int steps = 8;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     float best_ms = 0.f;
-    int best_bpg = b->bricks_per_group, best_nt = b->nontemporal;
+    int best_bpg = b->bricks_per_group, best_nt = b->nontemporal, best_cap = b->waves_cap;
     int rc = LBM355_OK;
     bool first = true;
-    for (int nt = 0; nt <= 1 && rc == LBM355_OK; ++nt)
-        for (int k = 0; k < 4 && rc == LBM355_OK; ++k) {
-            b->bricks_per_group = shapes[k];
-            b->nontemporal = nt;
-            drop_graph(*b);
-            if ((rc = batch_enqueue(b, 2)) || (rc = batch_finish(b))) break;  // warm-up (and graph capture)
-            if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
-            if ((rc = batch_enqueue(b, steps))) break;
-            if (hipEventRecord(e1, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
-            if ((rc = batch_finish(b))) break;
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            if (first || ms < best_ms) { best_ms = ms; best_bpg = shapes[k]; best_nt = nt; }
-            first = false;
-        }
+    for (int k = 0; k < kTuneConfigs && rc == LBM355_OK; ++k) {
+        const TuneConfig t = tune_config(k);
+        b->bricks_per_group = t.bpg;
+        b->nontemporal = t.nt;
+        b->waves_cap = t.cap;
+        drop_graph(*b);
+        if ((rc = batch_enqueue(b, 2)) || (rc = batch_finish(b))) break;  // warm-up (and graph capture)
+        if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
+        if ((rc = batch_enqueue(b, steps))) break;
+        if (hipEventRecord(e1, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
+        if ((rc = batch_finish(b))) break;
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (first || ms < best_ms) { best_ms = ms; best_bpg = t.bpg; best_nt = t.nt; best_cap = t.cap; }
+        first = false;
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     b->bricks_per_group = best_bpg;
     b->nontemporal = best_nt;
+    b->waves_cap = best_cap;
     drop_graph(*b);
-    if (best) *best = best_bpg | (best_nt << 8);
+    if (best) *best = best_bpg | (best_nt << 8) | (best_cap << 16);
     return LBM355_OK;
 }
 
 int lbm355_case_autotune(lbm355_case* c, int* best)
 {
     if (!c) return fail(LBM355_EINVAL, "null case");
-    const int shapes[4] = {1, 2, 4, 8};
     double best_ms = 0.0;
-    int best_bpg = c->bricks_per_group, best_nt = c->nontemporal;
+    int best_bpg = c->bricks_per_group, best_nt = c->nontemporal, best_cap = c->waves_cap;
     bool first = true;
     int rc;
-    for (int nt = 0; nt <= 1; ++nt)
-        for (int k = 0; k < 4; ++k) {
-            c->bricks_per_group = shapes[k];
-            c->nontemporal = nt;
-            if ((rc = lbm355_case_step(c, 2))) return rc;  // warm-up
-            double step_ms = 0.0;
-            if ((rc = lbm355_case_time_kernel(c, 8, &step_ms, nullptr))) return rc;
-            if (first || step_ms < best_ms) { best_ms = step_ms; best_bpg = shapes[k]; best_nt = nt; }
-            first = false;
-        }
+    for (int k = 0; k < kTuneConfigs; ++k) {
+        const TuneConfig t = tune_config(k);
+        c->bricks_per_group = t.bpg;
+        c->nontemporal = t.nt;
+        c->waves_cap = t.cap;
+        if ((rc = lbm355_case_step(c, 2))) return rc;  // warm-up
+        double step_ms = 0.0;
+        if ((rc = lbm355_case_time_kernel(c, 8, &step_ms, nullptr))) return rc;
+        if (first || step_ms < best_ms) { best_ms = step_ms; best_bpg = t.bpg; best_nt = t.nt; best_cap = t.cap; }
+        first = false;
+    }
     c->bricks_per_group = best_bpg;
     c->nontemporal = best_nt;
-    if (best) *best = best_bpg | (best_nt << 8);
+    c->waves_cap = best_cap;
+    if (best) *best = best_bpg | (best_nt << 8) | (best_cap << 16);
     return LBM355_OK;
 }
 

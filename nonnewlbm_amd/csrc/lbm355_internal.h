@@ -98,6 +98,7 @@ struct StepLaunch {
     int64_t rel;
     int32_t bricks_per_group; // 1, 2, 4, 8 wavefronts per workgroup
     int32_t nontemporal;      // 1: even-step population accesses with the nt hint
+    int32_t lds_bytes;        // dynamic LDS per workgroup (unused by the kernel): caps wavefronts per CU
 };
 
 // storage kinds (same values as LBM355_F64 ...)

@@ -1039,8 +1039,9 @@ static int launch_step_t(const StepLaunch& L, hipStream_t st)
     int groups = (L.n_list + bpg - 1) / bpg;
     groups = ((groups + 7) / 8) * 8;
     if (groups == 0) groups = 8;
-    if (L.nontemporal) hipLaunchKernelGGL((k_step<S, MODEL, true>), dim3(groups), dim3(64 * bpg), 0, st, L);
-    else hipLaunchKernelGGL((k_step<S, MODEL, false>), dim3(groups), dim3(64 * bpg), 0, st, L);
+    const unsigned lds = unsigned(L.lds_bytes);
+    if (L.nontemporal) hipLaunchKernelGGL((k_step<S, MODEL, true>), dim3(groups), dim3(64 * bpg), lds, st, L);
+    else hipLaunchKernelGGL((k_step<S, MODEL, false>), dim3(groups), dim3(64 * bpg), lds, st, L);
     return check_launch();
 }
 

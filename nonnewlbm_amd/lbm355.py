@@ -86,12 +86,17 @@ class Case:
         """-1 auto (by working set), 0 default cache policy, 1 non-temporal even step."""
         check(self.lib.lbm355_case_set_nontemporal(self.h, int(mode)), "case_set_nontemporal")
 
-    def autotune(self) -> tuple[int, int]:
-        """Time launch shapes x cache policy on 80 real steps; keeps and
-        returns the fastest (bricks_per_group, nontemporal)."""
+    def set_occupancy(self, waves_per_cu: int) -> None:
+        """Cap the fused step at waves_per_cu wavefronts per CU (4..32); 0 = no cap."""
+        check(self.lib.lbm355_case_set_occupancy(self.h, int(waves_per_cu)), "case_set_occupancy")
+
+    def autotune(self) -> tuple[int, int, int]:
+        """Time launch shapes x cache policy x wavefront cap on 240 real
+        steps; keeps and returns the fastest (bricks_per_group, nontemporal,
+        waves_per_cu)."""
         best = C.c_int()
         check(self.lib.lbm355_case_autotune(self.h, C.byref(best)), "case_autotune")
-        return best.value & 0xFF, best.value >> 8
+        return best.value & 0xFF, (best.value >> 8) & 0xFF, best.value >> 16
 
     def time_kernel(self, n: int) -> tuple[float, float]:
         """Run n steps; returns (ms per whole step, ms per fused-kernel launch)."""
@@ -179,11 +184,14 @@ class Batch:
     def set_shape(self, bricks_per_group: int) -> None:
         check(self.lib.lbm355_batch_set_shape(self.h, int(bricks_per_group)), "batch_set_shape")
 
-    def autotune(self) -> tuple[int, int]:
-        """Returns the fastest (bricks_per_group, nontemporal), now in use."""
+    def set_occupancy(self, waves_per_cu: int) -> None:
+        check(self.lib.lbm355_batch_set_occupancy(self.h, int(waves_per_cu)), "batch_set_occupancy")
+
+    def autotune(self) -> tuple[int, int, int]:
+        """Returns the fastest (bricks_per_group, nontemporal, waves_per_cu), now in use."""
         best = C.c_int()
         check(self.lib.lbm355_batch_autotune(self.h, C.byref(best)), "batch_autotune")
-        return best.value & 0xFF, best.value >> 8
+        return best.value & 0xFF, (best.value >> 8) & 0xFF, best.value >> 16
 
     def close(self) -> None:
         if getattr(self, "h", None):

@@ -61,23 +61,23 @@ def test_batch_then_alone_continues(gpu_lib):
 
 
 def test_autotune_keeps_results(gpu_lib):
-    """The autotuners time launch shapes and the cache policy on real steps;
-    the trajectory is the one of plain stepping."""
+    """The autotuners time launch shapes, the cache policy and the wavefront
+    cap on real steps; the trajectory is the one of plain stepping."""
     specs = _cases(_abi.F32)[:2]
     ref = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
     for c in ref:
-        c.step(100)
+        c.step(260)
     one = Case(specs[0][0].flat(), specs[0][0].dims, **specs[0][1])
-    bpg, nt = one.autotune()                   # 8 configurations x 10 steps
-    assert bpg in (1, 2, 4, 8) and nt in (0, 1)
-    assert one.info()["steps"] == 80
+    bpg, nt, cap = one.autotune()              # 24 configurations x 10 steps
+    assert bpg in (1, 2, 4, 8) and nt in (0, 1) and cap in (0, 16, 12)
+    assert one.info()["steps"] == 240
     one.step(20)
     for name in ("rho", "vx"):
         np.testing.assert_array_equal(one.field(name), ref[0].field(name))
     pair = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
     b = Batch(pair, graph=True)
-    bpg, nt = b.autotune()
-    assert bpg in (1, 2, 4, 8) and nt in (0, 1)
+    bpg, nt, cap = b.autotune()
+    assert bpg in (1, 2, 4, 8) and nt in (0, 1) and cap in (0, 16, 12)
     b.step(20)
     b.close()
     for k in range(2):

@@ -233,7 +233,8 @@ def test_output_strain_matches_oracle(gpu_lib):
 @pytest.mark.parametrize("storage", [_abi.F32, _abi.F16_OFFSET])
 def test_nontemporal_policy_changes_nothing(gpu_lib, storage):
     """The junction x2 (above the working set where the even step streams with
-    the non-temporal hint) gives the same bits with the hint forced on and off."""
+    the non-temporal hint) gives the same bits with the hint forced on and
+    off, and with and without a cap of 12 wavefronts per CU."""
     from nonnewlbm_amd.geometry import tcpc_mask
     g = tcpc_mask(2.0)
     kw = dict(storage=storage, collision=_abi.CUM, nu_ref=0.01, profile=_abi.PROFILE_PARABOLIC, ivc_umax=0.04,
@@ -241,6 +242,7 @@ def test_nontemporal_policy_changes_nothing(gpu_lib, storage):
               ivc_radius=g.meta["ivc"]["radius"], svc_radius=g.meta["svc"]["radius"], dl=g.dl, dt=1e-5)
     a = Case(g.flat(), g.dims, **kw)
     a.set_nontemporal(1)
+    a.set_occupancy(12)
     a.step(6)
     va = a.field("vx")
     a.close()
