@@ -211,6 +211,17 @@ def precision_study(scale: float, steps: int) -> dict:
     return res
 
 
+def configure(case, launch: str) -> tuple[int, int, int]:
+    """Autotune ("auto") or set 'bricks_per_group,nontemporal,waves_per_cu'."""
+    if launch == "auto":
+        return case.autotune()
+    bpg, nt, cap = (int(v) for v in launch.split(","))
+    case.set_shape(bpg)
+    case.set_nontemporal(nt)
+    case.set_occupancy(cap)
+    return bpg, nt, cap
+
+
 def cpu_baseline(g, kw, budget_s: float) -> dict:
     from oracle.oracle import OracleCase
 
@@ -244,6 +255,11 @@ def main() -> int:
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
                     help="also run the FP64/FP32/FP16/BF16 storage study for STEPS steps")
     ap.add_argument("--precision-scale", type=float, default=2.0)
+    ap.add_argument("--launch", default="auto", metavar="BPG,NT,CAP",
+                    help="launch configuration of the 256x192x128 line (bricks per workgroup, non-temporal even "
+                         "step, wavefronts per CU); default: autotune")
+    ap.add_argument("--launch-hbm", default="auto", metavar="BPG,NT,CAP",
+                    help="the same for the 512x384x256 line")
     ap.add_argument("--sweep-requests", type=int, default=48)
     ap.add_argument("--sweep-rate", type=float, default=12.0, help="Poisson arrivals per second")
     ap.add_argument("--sweep-tick", type=int, default=64, help="steps per pool tick")
@@ -270,10 +286,11 @@ def main() -> int:
           f"device={info['device_bytes'] / 2**20:.1f} MiB", file=sys.stderr, flush=True)
 
     case.step(args.warmup)
-    # launch shape and even-step cache policy: the fastest of real steps
-    bpg, nt, cap = case.autotune()
+    # launch shape, even-step cache policy and wavefront cap: the fastest of
+    # real steps, or the given configuration
+    bpg, nt, cap = configure(case, args.launch)
     case.sync()
-    print(f"# autotune: {bpg} bricks per workgroup, non-temporal even step {nt}, wavefront cap {cap}",
+    print(f"# launch: {bpg} bricks per workgroup, non-temporal even step {nt}, wavefront cap {cap} ({args.launch})",
           file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     case.step(args.steps)
@@ -349,7 +366,7 @@ def main() -> int:
         g2, kw2 = make_case(2.0, storage, collision, args.lbm_viscosity)
         c2 = Case(g2.flat(), g2.dims, **kw2)
         i2 = c2.info()
-        bpg2, nt2, cap2 = c2.autotune()
+        bpg2, nt2, cap2 = configure(c2, args.launch_hbm)
         c2.step(50)
         s2, k2 = c2.time_kernel(400)
         b2 = i2["active_cells"] * 2 * 27 * sbytes
