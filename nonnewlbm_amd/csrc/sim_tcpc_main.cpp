@@ -54,6 +54,7 @@ struct Options {
     int stats_every = 1;
     double max_wall_time = -1.0;   // the reference's wallTime (state.h:256), seconds
     bool save_at_end = false;
+    bool autotune = true;
 };
 
 struct Material {
@@ -105,6 +106,8 @@ void usage()
               << "  --max-wall-time S                      save results_<id>/current_state and stop after S\n"
               << "                                         seconds (creates the loadstate flag file)\n"
               << "  --save-state-at-end                    save results_<id>/current_state when the run ends\n"
+              << "  --no-autotune                          keep the default launch configuration (by default it\n"
+              << "                                         is timed on a copy of the case before the run)\n"
               << "Restart: a file results_<id>/loadstate loads results_<id>/current_state at start;\n"
               << "a file results_<id>/savestate saves it at the next print event.\n";
 }
@@ -142,6 +145,7 @@ Options parse_args(int argc, char** argv)
         else if (arg == "--stats-every") o.stats_every = std::atoi(value("--stats-every"));
         else if (arg == "--max-wall-time") o.max_wall_time = std::atof(value("--max-wall-time"));
         else if (arg == "--save-state-at-end") o.save_at_end = true;
+        else if (arg == "--no-autotune") o.autotune = false;
         else throw std::runtime_error("Unknown argument '" + arg + "'");
     }
     return o;
@@ -466,6 +470,22 @@ int run(const Options& opt)
           opt.precision.c_str(), d.cy_enabled ? opt.cy_model.c_str() : "off", opt.profile.c_str(),
           (long long)info.active_cells, (long long)info.bricks_stored, double(info.device_bytes) / 1048576.0);
     const bool terminate = info.terminate != 0;
+    if (opt.autotune && !terminate) {
+        // the launch configuration is timed on a throwaway copy of the case,
+        // so the run's own trajectory, counters and outputs are untouched
+        lbm355_case* t = nullptr;
+        int best = 0;
+        if (lbm355_case_create(&d, &t) == LBM355_OK && lbm355_case_autotune(t, &best) == LBM355_OK) {
+            const int bpg = best & 0xFF, nt = (best >> 8) & 0xFF, cap = best >> 16;
+            check(lbm355_case_set_shape(c, bpg), "launch shape");
+            check(lbm355_case_set_nontemporal(c, nt), "cache policy");
+            check(lbm355_case_set_occupancy(c, cap), "wavefront cap");
+            L.log("lbm355: launch bricks_per_group=%d nontemporal=%d waves_per_cu=%d (timed on a copy)", bpg, nt, cap);
+        } else {
+            L.log("lbm355: launch autotune skipped (%s)", lbm355_last_error());
+        }
+        if (t) lbm355_case_destroy(t);
+    }
 
     Counter print_c, vtk_c, reset_c;
     print_c.period = opt.print_period;
