@@ -157,6 +157,145 @@ def sweep(n: int, rate: float, tick: int) -> dict:
     }
 
 
+def batch15(steps: int) -> dict:
+    """Fifteen independent junctions stepped as one batch (one launch per
+    step over their concatenated brick lists, graph replay) against the same
+    cases stepped one after another, each with the plain single-case launch:
+    5 parabolic inflow pairs u_max ~ U[0.01, 0.06] x {Newtonian 0.0035 Pa s,
+    CY-A, CY-B}, grid scale per case from {0.5, 0.75, 1.0, 1.5}
+    (numpy default_rng(0)), cumulant, FP32."""
+    from nonnewlbm_amd import _abi
+    from nonnewlbm_amd.geometry import tcpc_mask
+    from nonnewlbm_amd.lbm355 import Batch, Case
+    from nonnewlbm_amd.materials import case_kwargs, material_by_name
+    rng = np.random.default_rng(0)
+    pairs = [(float(rng.uniform(0.01, 0.06)), float(rng.uniform(0.01, 0.06))) for _ in range(5)]
+    geoms: dict = {}
+    specs = []
+    for umax in pairs:
+        for mat in ("blood-newtonian", "cy-a", "cy-b"):
+            sc = float(rng.choice([0.5, 0.75, 1.0, 1.5]))
+            if sc not in geoms:
+                geoms[sc] = tcpc_mask(sc)
+            g = geoms[sc]
+            kw = case_kwargs(material_by_name(mat), g.dl, 0.01)
+            kw.update(storage=_abi.F32, collision=_abi.CUM, profile=_abi.PROFILE_PARABOLIC, ivc_umax=umax[0],
+                      svc_umax=umax[1], ivc_center=g.meta["ivc"]["center"], svc_center=g.meta["svc"]["center"],
+                      ivc_radius=g.meta["ivc"]["radius"], svc_radius=g.meta["svc"]["radius"], stats_every=1)
+            specs.append((sc, g, kw))
+    # sequential: one case at a time, plain launches
+    seq_s, cells = 0.0, 0
+    for sc, g, kw in specs:
+        c = Case(g.flat(), g.dims, **kw)
+        c.step(2)
+        c.sync()
+        t0 = time.perf_counter()
+        c.step(steps)
+        c.sync()
+        seq_s += time.perf_counter() - t0
+        cells += c.info()["active_cells"]
+        c.close()
+    # batched
+    cases = [Case(g.flat(), g.dims, **kw) for sc, g, kw in specs]
+    b = Batch(cases, graph=True)
+    b.step(2)
+    t0 = time.perf_counter()
+    b.step(steps)
+    bat_s = time.perf_counter() - t0
+    # the same batch as one group: every step sweeps all fifteen cases
+    b.set_group_bytes(0)
+    b.step(2)
+    t0 = time.perf_counter()
+    b.step(steps)
+    one_s = time.perf_counter() - t0
+    finite = all(bool(np.isfinite(c.field("vx")).all()) for c in cases)
+    b.close()
+    for c in cases:
+        c.close()
+    seq = cells * steps / seq_s / 1e6
+    bat = cells * steps / bat_s / 1e6
+    print(f"# batch of 15: {bat:.0f} fluid MLUPS batched, {seq:.0f} sequential ({bat / seq:.2f}x)", file=sys.stderr,
+          flush=True)
+    small = small_batch(steps)
+    return {"workload": "15 TCPC junctions (5 parabolic u_max pairs ~ U[0.01,0.06] x Newtonian/CY-A/CY-B, scale "
+                        "per case from 0.5/0.75/1.0/1.5, default_rng(0)), cumulant, FP32",
+            "scales": [sc for sc, g, kw in specs], "fluid_cells": cells, "steps": steps,
+            "batched_fluid_mlups": round(bat, 1), "sequential_fluid_mlups": round(seq, 1),
+            "speedup": round(bat / seq, 3),
+            "one_group_fluid_mlups": round(cells * steps / one_s / 1e6, 1),
+            "grouping": "consecutive members up to 256 MiB of device memory per group, 64 steps per group in turn",
+            "results_finite": finite, "small_cases": small}
+
+
+def small_batch(steps: int) -> dict:
+    """Sixteen 128x96x64 junctions (44,420 fluid cells each, CY-A, cumulant,
+    FP32, u_max pairs ~ U[0.01, 0.06], default_rng(1)): one such case leaves
+    most of the GPU idle, a batch fills it."""
+    from nonnewlbm_amd import _abi
+    from nonnewlbm_amd.geometry import tcpc_mask
+    from nonnewlbm_amd.lbm355 import Batch, Case
+    from nonnewlbm_amd.materials import case_kwargs, material_by_name
+    rng = np.random.default_rng(1)
+    g = tcpc_mask(0.5)
+    specs = []
+    for _ in range(16):
+        kw = case_kwargs(material_by_name("cy-a"), g.dl, 0.01)
+        kw.update(storage=_abi.F32, collision=_abi.CUM, profile=_abi.PROFILE_PARABOLIC,
+                  ivc_umax=float(rng.uniform(0.01, 0.06)), svc_umax=float(rng.uniform(0.01, 0.06)),
+                  ivc_center=g.meta["ivc"]["center"], svc_center=g.meta["svc"]["center"],
+                  ivc_radius=g.meta["ivc"]["radius"], svc_radius=g.meta["svc"]["radius"], stats_every=1)
+        specs.append(kw)
+    cases = [Case(g.flat(), g.dims, **kw) for kw in specs]
+    cells = sum(c.info()["active_cells"] for c in cases)
+    seq_s = 0.0
+    for c in cases:
+        c.step(2)
+        c.sync()
+        t0 = time.perf_counter()
+        c.step(steps)
+        c.sync()
+        seq_s += time.perf_counter() - t0
+    b = Batch(cases, graph=True)
+    b.step(2)
+    t0 = time.perf_counter()
+    b.step(steps)
+    bat_s = time.perf_counter() - t0
+    b.close()
+    for c in cases:
+        c.close()
+    seq = cells * steps / seq_s / 1e6
+    bat = cells * steps / bat_s / 1e6
+    print(f"# batch of 16 small: {bat:.0f} fluid MLUPS batched, {seq:.0f} sequential ({bat / seq:.2f}x)",
+          file=sys.stderr, flush=True)
+    return {"workload": "16 TCPC junctions 128x96x64, CY-A, cumulant, FP32", "fluid_cells": cells, "steps": steps,
+            "batched_fluid_mlups": round(bat, 1), "sequential_fluid_mlups": round(seq, 1),
+            "speedup": round(bat / seq, 3)}
+
+
+def models(steps: int) -> dict:
+    """The 256x192x128 junction workload with each collision model (CY-A neq,
+    FP32, statistics every step), autotuned launch."""
+    from nonnewlbm_amd import _abi
+    from nonnewlbm_amd.lbm355 import Case
+    out = {}
+    for name, coll in (("BGK", _abi.BGK), ("MRT_LES", _abi.MRT), ("KBC_N4", _abi.KBC), ("CUM", _abi.CUM)):
+        g, kw = make_case(1.0, _abi.F32, coll, 0.01)
+        c = Case(g.flat(), g.dims, **kw)
+        c.step(50)
+        bpg, nt, cap = c.autotune()
+        c.sync()
+        t0 = time.perf_counter()
+        c.step(steps)
+        c.sync()
+        dt = time.perf_counter() - t0
+        active = c.info()["active_cells"]
+        out[name] = {"fluid_mlups": round(active * steps / dt / 1e6, 1), "step_us": round(dt / steps * 1e6, 2),
+                     "launch": [bpg, nt, cap]}
+        c.close()
+    print(f"# models: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def precision_study(scale: float, steps: int) -> dict:
     """Storage precision against FP64 on the pulsatile junction: cumulant +
     CY-A (neq), parabolic inflows 0.04/0.03 with u(t) = u (1 + 0.3 sin(2 pi t /
@@ -260,6 +399,10 @@ def main() -> int:
                          "step, wavefronts per CU); default: autotune")
     ap.add_argument("--launch-hbm", default="auto", metavar="BPG,NT,CAP",
                     help="the same for the 512x384x256 line")
+    ap.add_argument("--batch-steps", type=int, default=2000,
+                    help="steps of the 15-junction batch against sequential stepping (0: skip)")
+    ap.add_argument("--model-steps", type=int, default=2000,
+                    help="steps per collision model on the 256x192x128 workload (0: skip)")
     ap.add_argument("--sweep-requests", type=int, default=48)
     ap.add_argument("--sweep-rate", type=float, default=12.0, help="Poisson arrivals per second")
     ap.add_argument("--sweep-tick", type=int, default=64, help="steps per pool tick")
@@ -384,6 +527,12 @@ def main() -> int:
         print(f"# HBM-resident: k_step {s2 * 1e3:.1f} us -> {a2:.0f} GB/s ({a2 / PEAK_GBS:.1%})", file=sys.stderr,
               flush=True)
         c2.close()
+
+    if args.model_steps > 0:
+        out["models"] = models(args.model_steps)
+
+    if args.batch_steps > 0:
+        out["batch15"] = batch15(args.batch_steps)
 
     if not args.no_sweep:
         out["sweep"] = sweep(args.sweep_requests, args.sweep_rate, args.sweep_tick)

@@ -279,13 +279,21 @@ int lbm355_batch_destroy(lbm355_batch* b);
 /* launch shape: 4x4x4 bricks (wavefronts) per workgroup, 1, 2, 4 or 8 */
 int lbm355_batch_set_shape(lbm355_batch* b, int bricks_per_group);
 int lbm355_batch_set_occupancy(lbm355_batch* b, int waves_per_cu);
+/* members are stepped in groups of consecutive cases holding at most `bytes`
+   of device memory (a larger case is a group of its own; 0 = one group); each
+   group runs a whole chunk of steps before the next, so a group that fits the
+   256 MB Infinity Cache stays resident.  Default 256 MiB.  Results do not
+   depend on it.  No counterpart in the reference. */
+int lbm355_batch_set_group_bytes(lbm355_batch* b, int64_t bytes);
 /* times every launch shape (1, 2, 4, 8 bricks per workgroup), with and
    without the non-temporal even step, and with no cap, 16 or 12 wavefronts per
-   CU, on 10 real steps each (2 warm-up + 8 timed; they count as simulation
-   steps of every member) and keeps the fastest;
-   *best = bricks_per_group | nontemporal << 8 | waves_per_cu << 16 */
+   CU, on 10 real steps each (2 warm-up + 8 timed), then one group, 256 MiB
+   and 128 MiB groups on 18 steps each; all of them count as simulation steps
+   of every member (294 in total).  Keeps the fastest;
+   *best = bricks_per_group | nontemporal << 8 | waves_per_cu << 16 |
+           (group bytes >> 24) << 24 */
 int lbm355_batch_autotune(lbm355_batch* b, int* best);
-/* the same for one case (240 real steps) */
+/* the same for one case (240 real steps; no grouping) */
 int lbm355_case_autotune(lbm355_case* c, int* best);
 
 /* --- serving pool ----------------------------------------------------------

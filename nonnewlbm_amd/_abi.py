@@ -212,6 +212,7 @@ def load() -> C.CDLL:
         "lbm355_batch_destroy": (C.c_int, [P]),
         "lbm355_batch_set_shape": (C.c_int, [P, C.c_int]),
         "lbm355_batch_set_occupancy": (C.c_int, [P, C.c_int]),
+        "lbm355_batch_set_group_bytes": (C.c_int, [P, C.c_int64]),
         "lbm355_batch_autotune": (C.c_int, [P, C.POINTER(C.c_int)]),
         "lbm355_case_footprint": (C.c_int, [C.POINTER(CaseDesc), C.POINTER(C.c_int64)]),
         "lbm355_case_labels": (C.c_int, [C.POINTER(CaseDesc), C.POINTER(C.c_int32), C.POINTER(CaseInfo)]),
@@ -244,7 +245,7 @@ EXPORTED = [
     "lbm355_case_step", "lbm355_case_sync", "lbm355_case_set_shape", "lbm355_case_set_nontemporal", "lbm355_case_set_occupancy", "lbm355_case_autotune", "lbm355_case_get_field", "lbm355_case_get_populations",
     "lbm355_case_compute_strain", "lbm355_case_flux", "lbm355_case_reset_stats", "lbm355_case_write_vtk",
     "lbm355_batch_create", "lbm355_batch_step", "lbm355_batch_destroy", "lbm355_batch_set_shape",
-    "lbm355_batch_autotune", "lbm355_batch_set_occupancy", "lbm355_case_footprint", "lbm355_case_labels", "lbm355_case_time_kernel",
+    "lbm355_batch_autotune", "lbm355_batch_set_occupancy", "lbm355_batch_set_group_bytes", "lbm355_case_footprint", "lbm355_case_labels", "lbm355_case_time_kernel",
     "lbm355_copy_bandwidth",
     "lbm355_case_save_state", "lbm355_case_load_state",
     "lbm355_pool_create", "lbm355_pool_submit", "lbm355_pool_poll", "lbm355_pool_wait", "lbm355_pool_drain",

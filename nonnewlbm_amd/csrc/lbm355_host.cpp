@@ -1562,6 +1562,18 @@ struct lbm355_batch {
     hipStream_t stream = nullptr;
     DevBuf devcases, list, base;
     int n_list = 0;
+    // the cases are stepped in groups of consecutive members whose device
+    // memory stays below group_bytes (0: one group); each group runs a whole
+    // chunk of steps before the next, so a group that fits the Infinity Cache
+    // stays there for the chunk
+    struct Group {
+        int begin, end;   // brick list range
+        int64_t bytes;
+    };
+    std::vector<Group> groups;
+    std::vector<int> case_first;        // brick list offset of each member
+    std::vector<int64_t> case_bytes;    // device memory of each member
+    int64_t group_bytes = kBatchGroupBytes;
     int64_t launches = 0;     // steps done by the batch
     int64_t base_host = 0;    // mirror of the device launch base
     std::vector<int64_t> start;  // per case: launch index of its step 0
@@ -1590,19 +1602,36 @@ TuneConfig tune_config(int k)
     return TuneConfig{shapes[k % 4], (k / 4) % 2, caps[k / 8]};
 }
 
-StepLaunch batch_launch(lbm355_batch& b)
+// split the members into groups of consecutive cases below group_bytes; a
+// case larger than that is a group of its own
+void regroup(lbm355_batch& b)
 {
+    b.groups.clear();
+    const int n = int(b.cases.size());
+    for (int i = 0; i < n; ++i) {
+        const int end = i + 1 < n ? b.case_first[i + 1] : b.n_list;
+        if (!b.groups.empty() && (b.group_bytes <= 0 || b.groups.back().bytes + b.case_bytes[i] <= b.group_bytes)) {
+            b.groups.back().end = end;
+            b.groups.back().bytes += b.case_bytes[i];
+        } else {
+            b.groups.push_back({b.case_first[i], end, b.case_bytes[i]});
+        }
+    }
+}
+
+// one step of group g of a batch
+StepLaunch batch_launch(lbm355_batch& b, size_t g)
+{
+    const lbm355_batch::Group& G = b.groups[g];
     StepLaunch L{};
-    L.list = static_cast<const BrickRef*>(b.list.p);
-    L.n_list = b.n_list;
+    L.list = static_cast<const BrickRef*>(b.list.p) + G.begin;
+    L.n_list = G.end - G.begin;
     L.cases = static_cast<CaseDev*>(b.devcases.p);
     L.n_cases = int32_t(b.cases.size());
     L.base = static_cast<const int64_t*>(b.base.p);
     L.rel = b.launches - b.base_host;
     L.bricks_per_group = b.bricks_per_group;
-    int64_t bytes = 0;
-    for (const lbm355_case* c : b.cases) bytes += case_device_bytes(*c);
-    L.nontemporal = b.nontemporal >= 0 ? b.nontemporal : (bytes > kNontemporalBytes ? 1 : 0);
+    L.nontemporal = b.nontemporal >= 0 ? b.nontemporal : (G.bytes > kNontemporalBytes ? 1 : 0);
     L.lds_bytes = occupancy_lds(b.waves_cap, b.bricks_per_group);
     return L;
 }
@@ -1640,11 +1669,14 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
         cd.back().start = -c->steps;  // batch launch 0 is the case's step c->steps
         b->start.push_back(-c->steps);
         b->epoch.push_back(c->stats_first);
+        b->case_first.push_back(int(list.size()));
+        b->case_bytes.push_back(case_device_bytes(*c));
         for (int k = 0; k < c->n_active; ++k) list.push_back(BrickRef{uint32_t(i), uint32_t(k)});
     }
     // the cases' Morton-ordered brick runs follow each other; the
-    // XCD-contiguous split of the list gives each XCD consecutive bricks
+    // XCD-contiguous split of a group's range gives each XCD consecutive bricks
     b->n_list = int(list.size());
+    regroup(*b);
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     int rc;
     if ((rc = upload(b->devcases, cd))) return rc;
@@ -1665,6 +1697,15 @@ int lbm355_batch_set_shape(lbm355_batch* b, int bpg)
 {
     if (!b || !(bpg == 1 || bpg == 2 || bpg == 4 || bpg == 8)) return fail(LBM355_EINVAL, "bricks per group");
     b->bricks_per_group = bpg;
+    drop_graph(*b);
+    return LBM355_OK;
+}
+
+int lbm355_batch_set_group_bytes(lbm355_batch* b, int64_t bytes)
+{
+    if (!b || bytes < 0) return fail(LBM355_EINVAL, "group bytes must be >= 0");
+    b->group_bytes = bytes;
+    regroup(*b);
     drop_graph(*b);
     return LBM355_OK;
 }
@@ -1706,14 +1747,18 @@ int batch_enqueue(lbm355_batch* b, int64_t n)
     }
     int64_t done = 0;
     if (b->use_graph) {
-        const int chunk = 16;
+        // steps per group per replay: long enough that refilling the
+        // Infinity Cache at a group switch is a few percent of the chunk
+        const int chunk = 64;
         if (!b->exec) {
-            StepLaunch L = batch_launch(*b);
             HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             int r = 0;
-            for (int s = 0; s < chunk && r == 0; ++s) {
-                L.rel = s;
-                r = launch_step(b->storage, b->model, L, b->stream);
+            for (size_t g = 0; g < b->groups.size() && r == 0; ++g) {
+                StepLaunch L = batch_launch(*b, g);
+                for (int s = 0; s < chunk && r == 0; ++s) {
+                    L.rel = s;
+                    r = launch_step(b->storage, b->model, L, b->stream);
+                }
             }
             if (r == 0) r = launch_advance(static_cast<int64_t*>(b->base.p), chunk, b->stream);
             hipGraph_t g = nullptr;
@@ -1738,10 +1783,18 @@ int batch_enqueue(lbm355_batch* b, int64_t n)
             b->base_host += b->graph_steps;
         }
     }
-    for (; done < n; ++done) {
-        const StepLaunch L = batch_launch(*b);
-        LAUNCH_TRY(launch_step(b->storage, b->model, L, b->stream));
-        b->launches += 1;
+    // the remaining steps group by group; every group replays the same
+    // launch indices, so each member sees its own consecutive steps
+    if (n > done) {
+        for (size_t g = 0; g < b->groups.size(); ++g) {
+            StepLaunch L = batch_launch(*b, g);
+            const int64_t rel0 = L.rel;
+            for (int64_t s = 0; s < n - done; ++s) {
+                L.rel = rel0 + s;
+                LAUNCH_TRY(launch_step(b->storage, b->model, L, b->stream));
+            }
+        }
+        b->launches += n - done;
     }
     for (lbm355_case* c : b->cases) {
         if (c->d.stats_every > 0) {
@@ -1803,6 +1856,31 @@ int lbm355_batch_autotune(lbm355_batch* b, int* best)
         if (first || ms < best_ms) { best_ms = ms; best_bpg = t.bpg; best_nt = t.nt; best_cap = t.cap; }
         first = false;
     }
+    if (rc == LBM355_OK) {
+        // then the grouping, with the launch configuration just chosen
+        b->bricks_per_group = best_bpg;
+        b->nontemporal = best_nt;
+        b->waves_cap = best_cap;
+        const int64_t sizes[3] = {0, int64_t(256) << 20, int64_t(128) << 20};
+        int64_t best_group = b->group_bytes;
+        bool gfirst = true;
+        for (int k = 0; k < 3 && rc == LBM355_OK; ++k) {
+            b->group_bytes = sizes[k];
+            regroup(*b);
+            drop_graph(*b);
+            if ((rc = batch_enqueue(b, 2)) || (rc = batch_finish(b))) break;
+            if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
+            if ((rc = batch_enqueue(b, 2 * steps))) break;
+            if (hipEventRecord(e1, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
+            if ((rc = batch_finish(b))) break;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (gfirst || ms < best_ms) { best_ms = ms; best_group = sizes[k]; }
+            gfirst = false;
+        }
+        b->group_bytes = best_group;
+        regroup(*b);
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
@@ -1810,7 +1888,7 @@ int lbm355_batch_autotune(lbm355_batch* b, int* best)
     b->nontemporal = best_nt;
     b->waves_cap = best_cap;
     drop_graph(*b);
-    if (best) *best = best_bpg | (best_nt << 8) | (best_cap << 16);
+    if (best) *best = best_bpg | (best_nt << 8) | (best_cap << 16) | int(b->group_bytes >> 24) << 24;
     return LBM355_OK;
 }
 

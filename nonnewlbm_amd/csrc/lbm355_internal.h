@@ -87,6 +87,8 @@ struct BrickRef {
 // the non-temporal hint (measured crossover on MI355X between 331 MB, where the
 // Infinity Cache still serves the next step, and 509 MB)
 constexpr int64_t kNontemporalBytes = int64_t(448) << 20;
+// batches step their members in groups of at most this much device memory
+constexpr int64_t kBatchGroupBytes = int64_t(256) << 20;
 
 // launch descriptor of one fused step over a brick list
 struct StepLaunch {

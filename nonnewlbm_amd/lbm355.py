@@ -187,11 +187,17 @@ class Batch:
     def set_occupancy(self, waves_per_cu: int) -> None:
         check(self.lib.lbm355_batch_set_occupancy(self.h, int(waves_per_cu)), "batch_set_occupancy")
 
-    def autotune(self) -> tuple[int, int, int]:
-        """Returns the fastest (bricks_per_group, nontemporal, waves_per_cu), now in use."""
+    def set_group_bytes(self, nbytes: int) -> None:
+        """Step members in groups of at most nbytes of device memory (0: one group)."""
+        check(self.lib.lbm355_batch_set_group_bytes(self.h, int(nbytes)), "batch_set_group_bytes")
+
+    def autotune(self) -> tuple[int, int, int, int]:
+        """Returns the fastest (bricks_per_group, nontemporal, waves_per_cu,
+        group MiB with 0 = one group), now in use; 294 real steps."""
         best = C.c_int()
         check(self.lib.lbm355_batch_autotune(self.h, C.byref(best)), "batch_autotune")
-        return best.value & 0xFF, (best.value >> 8) & 0xFF, best.value >> 16
+        v = best.value
+        return v & 0xFF, (v >> 8) & 0xFF, (v >> 16) & 0xFF, ((v >> 24) & 0xFF) * 16
 
     def close(self) -> None:
         if getattr(self, "h", None):

@@ -22,9 +22,12 @@ def _cases(storage):
     return out
 
 
+@pytest.mark.parametrize("group_bytes", [None, 1])
 @pytest.mark.parametrize("graph", [False, True])
 @pytest.mark.parametrize("storage", [_abi.F32, _abi.F16_OFFSET])
-def test_batch_equals_alone(gpu_lib, graph, storage):
+def test_batch_equals_alone(gpu_lib, graph, storage, group_bytes):
+    """group_bytes=1 steps every member as a group of its own (each group runs
+    its chunk of steps before the next one starts)."""
     specs = _cases(storage)
     alone = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
     mixed = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
@@ -33,6 +36,8 @@ def test_batch_equals_alone(gpu_lib, graph, storage):
     # one case starts earlier: the batch must keep each case's own step index
     mixed[1].step(5)
     b = Batch(mixed, graph=graph)
+    if group_bytes is not None:
+        b.set_group_bytes(group_bytes)
     b.step(32)
     b.step(5)
     # case 1 is 5 steps ahead: step the others alone to 37 and compare
@@ -66,18 +71,18 @@ def test_autotune_keeps_results(gpu_lib):
     specs = _cases(_abi.F32)[:2]
     ref = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
     for c in ref:
-        c.step(260)
+        c.step(314)
     one = Case(specs[0][0].flat(), specs[0][0].dims, **specs[0][1])
     bpg, nt, cap = one.autotune()              # 24 configurations x 10 steps
     assert bpg in (1, 2, 4, 8) and nt in (0, 1) and cap in (0, 16, 12)
     assert one.info()["steps"] == 240
-    one.step(20)
+    one.step(74)
     for name in ("rho", "vx"):
         np.testing.assert_array_equal(one.field(name), ref[0].field(name))
     pair = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
     b = Batch(pair, graph=True)
-    bpg, nt, cap = b.autotune()
-    assert bpg in (1, 2, 4, 8) and nt in (0, 1) and cap in (0, 16, 12)
+    bpg, nt, cap, group = b.autotune()          # 24 x 10 + 3 x 18 steps
+    assert bpg in (1, 2, 4, 8) and nt in (0, 1) and cap in (0, 16, 12) and group in (0, 256, 128)
     b.step(20)
     b.close()
     for k in range(2):
