@@ -101,23 +101,80 @@ def _sweep_case(req: dict, geoms: dict, storage=None):
     return g, kw
 
 
-def sweep(n: int, rate: float, tick: int) -> dict:
+FULL_GRIDS = ((128, 128, 128), (192, 144, 96), (256, 192, 128), (384, 288, 192), (512, 384, 256))
+
+
+def sweep_full_requests(n: int = 200, rate: float = 2.0, seed: int = 0) -> list[dict]:
+    """The full serving sweep: Poisson arrivals at `rate` per second; IVC
+    radius R ~ U[10, 28] cells (SVC 0.8 R, pulmonary 0.7 R, the 20/16/14 ratios
+    of the 256x192x128 junction), caval offset ~ U[0, 12] cells, grid from
+    FULL_GRIDS (dl = 64 mm / X), collision in BGK/MRT/KBC/CUM, material in
+    Newtonian/CY-A/CY-B, FP32 or FP16-offset, 5,000-50,000 steps, parabolic
+    u_max ~ U[0.01, 0.06] (numpy default_rng(seed))."""
+    from nonnewlbm_amd import _abi
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        out.append(dict(arrival=t, radius=float(rng.uniform(10.0, 28.0)), offset=float(rng.uniform(0.0, 12.0)),
+                        grid=FULL_GRIDS[int(rng.integers(len(FULL_GRIDS)))],
+                        collision=int(rng.choice([_abi.BGK, _abi.MRT, _abi.KBC, _abi.CUM])),
+                        material=str(rng.choice(["blood-newtonian", "cy-a", "cy-b"])),
+                        storage=int(rng.choice([_abi.F32, _abi.F16_OFFSET])),
+                        steps=int(rng.integers(5000, 50001)),
+                        umax=(float(rng.uniform(0.01, 0.06)), float(rng.uniform(0.01, 0.06)))))
+    return out
+
+
+def _full_case(req: dict, storage=None):
+    """(int8 x-fastest wall mask, dims, case keywords) of a full-sweep request."""
+    from nonnewlbm_amd import _abi
+    from nonnewlbm_amd.geometry import tcpc_mask
+    from nonnewlbm_amd.materials import case_kwargs, material_by_name
+    X, Y, Z = req["grid"]
+    R = req["radius"]
+    g = tcpc_mask(1.0, r_ivc=R, r_svc=0.8 * R, r_pa=0.7 * R, offset=req["offset"], dims=(X, Y, Z), dl=0.064 / X)
+    kw = case_kwargs(material_by_name(req["material"]), g.dl, 0.01)
+    kw.update(storage=req["storage"] if storage is None else storage, collision=req["collision"],
+              profile=_abi.PROFILE_PARABOLIC, ivc_umax=req["umax"][0], svc_umax=req["umax"][1],
+              ivc_center=g.meta["ivc"]["center"], svc_center=g.meta["svc"]["center"],
+              ivc_radius=g.meta["ivc"]["radius"], svc_radius=g.meta["svc"]["radius"], stats_every=1)
+    return g.flat().astype(np.int8), g.dims, kw
+
+
+def sweep(n: int, rate: float, tick: int, full: bool = False, budget: int = 0, seq_sample: int = 0) -> dict:
     """Requests arrive on their Poisson schedule into one Pool; latency =
     completion - submission on the pool clock.  Baseline: the same requests
-    run back to back, one plain FP32 case at a time."""
+    run back to back, one plain FP32 case at a time (all of them, or the first
+    `seq_sample` for at most 4,000 steps each)."""
     from nonnewlbm_amd.lbm355 import Case, Pool
-    reqs = sweep_requests(n, rate)
-    geoms: dict = {}
-    specs = [_sweep_case(r, geoms) for r in reqs]
-    pool = Pool(steps_per_tick=tick, graph=True)
+    if full:
+        reqs = sweep_full_requests(n, rate)
+        specs = []
+        for k, r in enumerate(reqs):
+            specs.append(_full_case(r))
+            if (k + 1) % 20 == 0:
+                print(f"# sweep: {k + 1}/{n} geometries", file=sys.stderr, flush=True)
+    else:
+        reqs = sweep_requests(n, rate)
+        geoms: dict = {}
+        specs = []
+        for r in reqs:
+            g, kw = _sweep_case(r, geoms)
+            specs.append((g.flat(), g.dims, kw))
+    pool = Pool(budget_bytes=budget, steps_per_tick=tick, graph=True)
     t0 = time.perf_counter()
     ids = []
-    for r, (g, kw) in zip(reqs, specs):
+    for r, (flat, dims, kw) in zip(reqs, specs):
         wait = r["arrival"] - (time.perf_counter() - t0)
         if wait > 0:
             time.sleep(wait)
-        ids.append(pool.submit(g.flat(), g.dims, r["steps"], report_every=1000, **kw))
-    pool.drain()
+        ids.append(pool.submit(flat, dims, r["steps"], report_every=1000, **kw))
+    while not pool.drain(timeout=30.0):
+        info = pool.stats()
+        print(f"# sweep: {info['done']} done, {info['running']} running, {info['queued']} queued, "
+              f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
     sts = [pool.poll(i) for i in ids]
     info = pool.stats()
     pool.close()
@@ -128,23 +185,33 @@ def sweep(n: int, rate: float, tick: int) -> dict:
     updates = info["cell_updates"]
     print(f"# sweep: {len(done)}/{n} done, p50 {np.percentile(lat, 50):.3f} s, p99 {np.percentile(lat, 99):.3f} s, "
           f"{updates / info['busy_seconds'] / 1e6:.0f} fluid MLUPS while stepping", file=sys.stderr, flush=True)
-    # sequential plain FP32 baseline over the same requests
+    # sequential plain FP32 baseline over the same requests (or a sample)
     from nonnewlbm_amd import _abi
-    t1 = time.perf_counter()
+    seq_s = 0.0
     seq_updates = 0
-    for r in reqs:
-        g, kw = _sweep_case(r, geoms, storage=_abi.F32)
-        c = Case(g.flat(), g.dims, **kw)
-        c.step(r["steps"])
+    chosen = reqs if seq_sample <= 0 else reqs[:seq_sample]
+    for r, (flat, dims, kw) in zip(chosen, specs):
+        kw = dict(kw, storage=_abi.F32)
+        c = Case(flat, dims, **kw)
+        steps = r["steps"] if seq_sample <= 0 else min(r["steps"], 4000)
         c.sync()
-        seq_updates += c.info()["active_cells"] * r["steps"]
+        t1 = time.perf_counter()
+        c.step(steps)
+        c.sync()
+        seq_s += time.perf_counter() - t1
+        seq_updates += c.info()["active_cells"] * steps
         c.close()
-    seq_s = time.perf_counter() - t1
+    workload = ("TCPC junctions, IVC R ~ U[10,28], caval offset ~ U[0,12], grids 128^3 .. 512x384x256, "
+                "BGK/MRT/KBC/CUM, Newtonian/CY-A/CY-B, FP32/FP16-offset, parabolic u_max U[0.01,0.06], "
+                "5000-50000 steps" if full else
+                "TCPC junctions at scale 0.5/0.75/1.0 (128x96x64 .. 256x192x128), BGK/MRT/KBC/CUM, "
+                "Newtonian/CY-A/CY-B, FP32/FP16-offset, parabolic u_max U[0.01,0.06], 1000-6000 steps")
     return {
         "requests": n, "done": len(done), "failed": n - len(done), "arrival_rate_per_s": rate,
-        "steps_per_tick": tick, "seed": 0,
-        "workload": "TCPC junctions at scale 0.5/0.75/1.0 (128x96x64 .. 256x192x128), BGK/MRT/KBC/CUM, "
-                    "Newtonian/CY-A/CY-B, FP32/FP16-offset, parabolic u_max U[0.01,0.06], 1000-6000 steps",
+        "steps_per_tick": tick, "seed": 0, "budget_bytes": info["budget_bytes"],
+        "workload": workload,
+        "sequential_sample": "all requests" if seq_sample <= 0 else
+                             f"the first {seq_sample} requests, at most 4000 steps each",
         "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p95": round(float(np.percentile(lat, 95)), 4),
                       "p99": round(float(np.percentile(lat, 99)), 4), "max": round(float(lat.max()), 4)},
         "first_report_s": {"p50": round(float(np.percentile(first, 50)), 4),
@@ -403,6 +470,8 @@ def main() -> int:
                     help="steps of the 15-junction batch against sequential stepping (0: skip)")
     ap.add_argument("--model-steps", type=int, default=2000,
                     help="steps per collision model on the 256x192x128 workload (0: skip)")
+    ap.add_argument("--sweep-full", action="store_true",
+                    help="the 200-request sweep at its real sizes (several minutes) instead of the short one")
     ap.add_argument("--sweep-requests", type=int, default=48)
     ap.add_argument("--sweep-rate", type=float, default=12.0, help="Poisson arrivals per second")
     ap.add_argument("--sweep-tick", type=int, default=64, help="steps per pool tick")
@@ -534,7 +603,9 @@ def main() -> int:
     if args.batch_steps > 0:
         out["batch15"] = batch15(args.batch_steps)
 
-    if not args.no_sweep:
+    if args.sweep_full:
+        out["sweep_full"] = sweep(200, 2.0, args.sweep_tick, full=True, budget=240 * 10**9, seq_sample=16)
+    elif not args.no_sweep:
         out["sweep"] = sweep(args.sweep_requests, args.sweep_rate, args.sweep_tick)
 
     if args.precision_study > 0:

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/lbm355.h"
+#include "lbm355_internal.h"
 
 // from lbm355_host.cpp: batch stepping split into enqueue and wait, and the
 // library's thread-local error message
@@ -200,10 +201,27 @@ void lbm355_pool::run()
         n = std::max<int64_t>(n, 1);
         const auto t_tick = std::chrono::steady_clock::now();
         int rc = LBM355_OK;
-        for (lbm355_batch* b : batches)
+        // while all running cases fit the Infinity Cache together, the
+        // batches of a tick run concurrently on their streams; beyond that
+        // they run one after another, so that each batch's cache-sized groups
+        // do not evict each other
+        int64_t resident = 0;
+        for (Request* r : running) resident += r->st.device_bytes;
+        const bool serial = resident > lbm355::kBatchGroupBytes;
+        for (lbm355_batch* b : batches) {
             if (rc == LBM355_OK) rc = batch_enqueue(b, n);
-        for (lbm355_case* c : solo)
+            if (serial) {
+                const int r2 = batch_finish(b);
+                if (rc == LBM355_OK) rc = r2;
+            }
+        }
+        for (lbm355_case* c : solo) {
             if (rc == LBM355_OK) rc = lbm355_case_step(c, n);  // queued on the case's stream
+            if (serial) {
+                const int r2 = lbm355_case_sync(c);
+                if (rc == LBM355_OK) rc = r2;
+            }
+        }
         for (lbm355_batch* b : batches) {
             const int r2 = batch_finish(b);
             if (rc == LBM355_OK) rc = r2;
