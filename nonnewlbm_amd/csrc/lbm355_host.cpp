@@ -616,17 +616,23 @@ int launch_case_step(lbm355_case& c, const StepLaunch& L)
 int occupancy_lds(int cap, int bpg)
 {
     if (cap <= 0) return 0;
-    static int per_cu = 0, per_block = 0;
-    if (per_cu == 0) {
+    struct Lds {
+        int per_cu, per_block;
+    };
+    // queried once (thread-safe static initialisation); every GPU this
+    // library drives is the same model
+    static const Lds lds = [] {
         int dev = 0, v = 0;
+        Lds r{160 * 1024, 64 * 1024};
         (void)hipGetDevice(&dev);
-        per_cu = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess && v > 0
-                     ? v : 160 * 1024;
-        per_block = hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0
-                        ? v : 64 * 1024;
-    }
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess && v > 0)
+            r.per_cu = v;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            r.per_block = v;
+        return r;
+    }();
     const int groups = std::max(1, cap / bpg);
-    return std::min(per_cu / (groups + 1) + 1, per_block);
+    return std::min(lds.per_cu / (groups + 1) + 1, lds.per_block);
 }
 
 StepLaunch single_launch(lbm355_case& c)
