@@ -11,8 +11,12 @@
 // golden vectors, so this restatement is pinned by the numpy checks in
 // tests/test_oracle_pins.py: central moments by 27x27 matrices, the Gaussian
 // (Isserlis) closure of the cumulant collision, closed forms, conservation,
-// fixed points, the Chapman-Enskog stress of a known shear rate, and
-// brute-force transport on tiny lattices.
+// fixed points, the Chapman-Enskog stress of a known shear rate, the fd-force
+// body force of a quadratic velocity field in closed form, and brute-force
+// transport on tiny lattices.
+//
+// Parity unpinned at the bit level: the rounding sequence of the reference's
+// own cumulant and KBC code (the closed forms pin them to rounding only).
 //
 // Deviations from the reference, all deliberate:
 //  - statistics sums are kept in double for every storage type (the reference

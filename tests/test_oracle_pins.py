@@ -360,6 +360,40 @@ def test_fd_strain_exact_on_linear_fields():
         assert np.all(S[k][~fl] == 0.0)
 
 
+def test_fd_force_closed_form():
+    """fd-force body force (core.h:663-781) from a state with u_x = a (y - y0)^2:
+    central differences are exact on it, so S12 = a (y - y0), div S = (a, 0, 0),
+    gamma = 2 |a (y - y0)| and F = 2 (nu(gamma) - nu_ref) a with the textbook
+    Carreau-Yasuda nu, rho = 1, away from the walls."""
+    dims = (8, 16, 8)
+    mat = material_by_name("cy-a")
+    u = lattice_units(mat, 0.25e-3, 0.01)
+    kw = dict(dl=0.25e-3, dt=u.dt, nu_ref=0.01, cy_enabled=1, cy_model=_abi.CY_FD_FORCE, cy_nu0=u.nu0,
+              cy_lambda=u.lam, cy_a=u.a, cy_n=u.n)
+    case, _ = _box_case(dims, **kw)
+    lab = case.field("label")
+    a, y0 = 1.0e-3, 7.25
+    pops = np.empty((27,) + dims)
+    for y in range(dims[1]):
+        pops[:, :, y, :] = O.feq(1.0, a * (y - y0) ** 2, 0.0, 0.0)[:, None, None]
+    case.set_populations(pops)
+    case.step(1)
+    fx, fy, fz = (case.field(k) for k in ("fx", "fy", "fz"))
+    checked = 0
+    for x, y, z in zip(*np.nonzero(lab == 0)):
+        if not (2 <= x < dims[0] - 2 and 3 <= y < dims[1] - 3 and 2 <= z < dims[2] - 2):
+            continue
+        assert np.all(lab[x - 1:x + 2, y - 2:y + 3, z - 1:z + 2] == 0)
+        gam = 2.0 * abs(a * (y - y0))
+        nu = 0.01 + (u.nu0 - 0.01) * (1.0 + (gam * u.lam) ** u.a) ** ((u.n - 1.0) / u.a)
+        np.testing.assert_allclose(fx[x, y, z], 2.0 * (nu - 0.01) * a, rtol=1e-5, atol=0)
+        assert abs(fy[x, y, z]) < 1e-15 and abs(fz[x, y, z]) < 1e-15
+        checked += 1
+    assert checked >= 4 * 10 * 4
+    # the shear-thinning part is what the force carries here
+    assert 0.0 < fx[4, 3, 4] < fx[4, 7, 4]
+
+
 def test_label_precedence_and_extrusion():
     dims = (8, 7, 6)
     case, _ = _box_case(dims, face_bc=(0,) * 6)
