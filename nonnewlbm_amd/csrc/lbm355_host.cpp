@@ -191,6 +191,8 @@ struct lbm355_case {
     // the planes 0,1 and X-2,X-1 (and the same in y) share bricks: every BC
     // cell then lives in the wavefront of the cells whose slots it reads,
     // which lets the fused kernel read the BC partner populations itself.
+    // z needs no shift: both z faces are always walls, so no BC cell has a
+    // z-clamped partner (setup_device checks the property cell by cell).
     int ox = 0, oy = 0;
     std::vector<int32_t> brick_id;      // brick grid -> storage index or -1
     std::vector<int32_t> brick_xyz;     // storage index -> (bx,by,bz)
@@ -466,6 +468,18 @@ int setup_device(lbm355_case& c)
             c.bc_label_h.push_back(uint8_t(lb));
             int src[kQ][3];
             partner_sources(c, x, y, z, lb, src);
+            // the slot read for q is rewritten in the same step by cell
+            // C_q + c_q (either parity); that cell must be a lane of this
+            // wavefront so that the read precedes the write.  The brick origin
+            // shift guarantees it on x and y faces, and z faces are walls.
+            for (int q = 0; q < kQ; ++q) {
+                const int wx = src[q][0] + qx(q), wy = src[q][1] + qy(q), wz = src[q][2] + qz(q);
+                if (wx < 0 || wy < 0 || wz < 0 || wx >= X || wy >= Y || wz >= Z) continue;
+                if (L.map[L.at(wx, wy, wz)] != L_WALL && c.stored_brick(wx, wy, wz) != b)
+                    return fail(LBM355_EINVAL, "boundary cell (" + std::to_string(x) + "," + std::to_string(y) + "," +
+                                                   std::to_string(z) +
+                                                   "): a partner slot is written outside its wavefront");
+            }
             // parity 0 (this step even, the previous one odd): slot q of C_q + c_q
             for (int q = 0; q < kQ; ++q)
                 bc_read.push_back(c.slot_offset(src[q][0] + qx(q), src[q][1] + qy(q), src[q][2] + qz(q), q));
