@@ -87,3 +87,29 @@ def test_pool_admission_budget_and_failures(gpu_lib):
     with pytest.raises(_abi.LbmError):
         pool.take(ids[0])                        # not submitted with keep
     pool.close()
+
+
+def test_pool_large_cases_step_batches_in_turn(gpu_lib):
+    """Running cases above the Infinity Cache (3 x 256x192x128 junctions,
+    ~100 MB each, two collision types): the pool steps the batches of a tick
+    one after another; every request still equals its case run alone."""
+    specs = []
+    for collision, steps, flux in ((_abi.CUM, 40, 1.0), (_abi.BGK, 33, 0.8), (_abi.CUM, 21, 1.2)):
+        g, kw = tcpc_case(1.0, storage=_abi.F32, collision=collision)
+        kw["ivc_flux"] *= flux
+        specs.append((g, kw, steps))
+    pool = Pool(steps_per_tick=16, graph=True)
+    ids = [pool.submit(g.flat(), g.dims, steps, keep=True, **kw) for g, kw, steps in specs]
+    assert pool.drain(timeout=120.0)
+    assert sum(pool.poll(r)["device_bytes"] for r in ids) > 256 << 20
+    for rid, (g, kw, steps) in zip(ids, specs):
+        st = pool.poll(rid)
+        assert st["state"] == _abi.REQ_DONE and st["steps_done"] == steps, st
+        got = pool.take(rid)
+        alone = Case(g.flat(), g.dims, **kw)
+        alone.step(steps)
+        for name in ("rho", "vx", "sum_rux"):
+            np.testing.assert_array_equal(got.field(name), alone.field(name), err_msg=f"{rid} {name}")
+        got.close()
+        alone.close()
+    pool.close()
