@@ -187,7 +187,7 @@ struct lbm355_case {
     Labels lab;
     int storage = 1, model = 0;
     int nbx = 0, nby = 0, nbz = 0;
-    // brick origin shift: cell x sits in brick (x + ox) / 4.  Chosen so that
+    // brick origin shift: cell x sits in brick (x + ox) / kBX.  Chosen so that
     // the planes 0,1 and X-2,X-1 (and the same in y) share bricks: every BC
     // cell then lives in the wavefront of the cells whose slots it reads,
     // which lets the fused kernel read the BC partner populations itself.
@@ -224,15 +224,15 @@ struct lbm355_case {
     int32_t stored_brick(int x, int y, int z) const
     {
         if (x < 0 || y < 0 || z < 0 || x >= d.nx || y >= d.ny || z >= d.nz) return -1;
-        return brick_id[((x + ox) >> 2) + int64_t(nbx) * (((y + oy) >> 2) + int64_t(nby) * (z >> 2))];
+        return brick_id[((x + ox) / kBX) + int64_t(nbx) * (((y + oy) / kBY) + int64_t(nby) * (z / kBZ))];
     }
     // element offset of slot `slot` of cell (x,y,z); the dummy brick if absent
-    int lane_of(int x, int y, int z) const { return ((x + ox) & 3) | (((y + oy) & 3) << 2) | ((z & 3) << 4); }
+    int lane_of(int x, int y, int z) const { return brick_lane(x + ox, y + oy, z); }
     void cell_of(int b, int l, int& x, int& y, int& z) const
     {
-        x = brick_xyz[3 * b] * 4 + (l & 3) - ox;
-        y = brick_xyz[3 * b + 1] * 4 + ((l >> 2) & 3) - oy;
-        z = brick_xyz[3 * b + 2] * 4 + (l >> 4);
+        x = brick_xyz[3 * b] * kBX + (l & (kBX - 1)) - ox;
+        y = brick_xyz[3 * b + 1] * kBY + ((l >> kSX) & (kBY - 1)) - oy;
+        z = brick_xyz[3 * b + 2] * kBZ + (l >> (kSX + kSY));
     }
     int32_t slot_offset(int x, int y, int z, int slot) const
     {
@@ -310,20 +310,22 @@ int build_index(lbm355_case& c)
     const lbm355_case_desc& d = c.d;
     const int X = d.nx, Y = d.ny, Z = d.nz;
     const Labels& L = c.lab;
-    auto pick_shift = [](int n) {
-        for (int o = 0; o < 3; ++o)
-            if ((n - 1 + o) % 4 != 0) return o;
+    // planes 0,1 share a brick for any shift o < B - 1; planes n-2, n-1 do
+    // unless n - 1 + o starts a brick
+    auto pick_shift = [](int n, int B) {
+        for (int o = 0; o < 3 && o < B - 1; ++o)
+            if ((n - 1 + o) % B != 0) return o;
         return 0;
     };
-    c.ox = pick_shift(X);
-    c.oy = pick_shift(Y);
-    c.nbx = (X + c.ox + 3) / 4; c.nby = (Y + c.oy + 3) / 4; c.nbz = (Z + 3) / 4;
+    c.ox = pick_shift(X, kBX);
+    c.oy = pick_shift(Y, kBY);
+    c.nbx = (X + c.ox + kBX - 1) / kBX; c.nby = (Y + c.oy + kBY - 1) / kBY; c.nbz = (Z + kBZ - 1) / kBZ;
     const int64_t NB = int64_t(c.nbx) * c.nby * c.nbz;
     // bricks touched by the AA update: the 3x3x3 neighbourhood of every
     // non-wall cell, plus the partner reads of BC cells
     std::vector<uint8_t> active_b(NB, 0), stored_b(NB, 0);
     auto bidx = [&](int x, int y, int z) {
-        return ((x + c.ox) >> 2) + int64_t(c.nbx) * (((y + c.oy) >> 2) + int64_t(c.nby) * (z >> 2));
+        return ((x + c.ox) / kBX) + int64_t(c.nbx) * (((y + c.oy) / kBY) + int64_t(c.nby) * (z / kBZ));
     };
     for (int z = 0; z < Z; ++z)
         for (int y = 0; y < Y; ++y)

@@ -2,13 +2,14 @@
 // driver of liblbm355.
 //
 // Data layout in HBM (per case)
-//   The grid is cut into 4x4x4 bricks; one brick = 64 cells = one wavefront.
+//   The grid is cut into kBX x kBY x kBZ bricks (16 x 4 x 1); one brick = 64
+//   cells = one wavefront.
 //   Bricks are stored only where the AA update touches memory (the non-wall
 //   cells and their 26-neighbourhood); active bricks (holding at least one
 //   non-wall cell) come first, in Morton order of the brick coordinates, then
 //   the storage-only shell bricks, then one "dummy" brick that stands in for
 //   every brick that is not stored.
-//   populations : f[brick][q][lane], lane = i + 4 j + 16 k, one 64-element
+//   populations : f[brick][q][lane], lane = i + kBX (j + kBY k), one 64-element
 //                 run per (brick, q): every wavefront load is one contiguous
 //                 64 x sizeof(T) segment (256 B at FP32).
 //   AA pattern  : one buffer.  An even step reads and writes the cell's own 27
@@ -23,8 +24,22 @@
 
 namespace lbm355 {
 
-constexpr int kBrick = 4;            // cells per brick edge
 constexpr int kLanes = 64;           // cells per brick = wavefront width
+// brick shape.  A 64-byte segment of an FP32 population run is one x row of
+// 16 cells, so the odd AA step's shifted accesses split segments only for
+// directions with c_x != 0 (with 4x4x4 bricks every c_x or c_y shift does).
+// Each axis that carries BC faces (x, y) needs at least 3 cells for the brick
+// origin shift; the z faces are always walls, so one cell suffices there.
+constexpr int kBX = 16, kBY = 4, kBZ = 1;
+constexpr int kSX = 4, kSY = 2;      // log2 kBX, log2 kBY
+static_assert(kBX * kBY * kBZ == kLanes && (1 << kSX) == kBX && (1 << kSY) == kBY, "brick shape");
+static_assert(kBX >= 3 && kBY >= 3, "x and y carry BC faces");
+
+// lane of brick-local cell (i, j, k), wrapped into the brick
+__host__ __device__ constexpr int brick_lane(int i, int j, int k)
+{
+    return (i & (kBX - 1)) | ((j & (kBY - 1)) << kSX) | ((k & (kBZ - 1)) << (kSX + kSY));
+}
 constexpr int kQ = 27;               // D3Q27
 constexpr int kStatFields = 6;       // sum rho u (3), sum (rho u)^2 (3)
 constexpr int kMacroFields = 4;      // rho, ux, uy, uz

@@ -1,6 +1,6 @@
 // lbm355_kernels.hip - gfx950 kernels of the D3Q27 step behind sim_tcpc.
 //
-// One wavefront = one 4x4x4 brick = 64 cells.  The fused kernel reads the 27
+// One wavefront = one brick of 64 cells.  The fused kernel reads the 27
 // incoming populations of each non-wall cell with the AA access pattern,
 // computes rho/u, applies the collision operator (or the boundary equilibrium
 // on BC cells) and writes the 27 outgoing populations back into the same
@@ -616,7 +616,7 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
 #pragma unroll
     for (int q = 0; q < 27; ++q) t[q] = nbr_c[q];
 
-    const int i = lane & 3, j = (lane >> 2) & 3, k = lane >> 4;
+    const int i = lane & (kBX - 1), j = (lane >> kSX) & (kBY - 1), k = lane >> (kSX + kSY);
     const uint32_t own = uint32_t(t[13]) + uint32_t(lane);
 
     R f[27];
@@ -631,8 +631,9 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
             for (int q = 0; q < 27; ++q) {
                 const int cx = qx(q), cy = qy(q), cz = qz(q);
                 const int si = i - cx, sj = j - cy, sk = k - cz;
-                const int nb = pick_brick(t, -cx, -cy, -cz, (si & ~3) != 0, (sj & ~3) != 0, (sk & ~3) != 0);
-                const int l2 = (si & 3) | ((sj & 3) << 2) | ((sk & 3) << 4);
+                const int nb = pick_brick(t, -cx, -cy, -cz, (si & ~(kBX - 1)) != 0, (sj & ~(kBY - 1)) != 0,
+                                          (sk & ~(kBZ - 1)) != 0);
+                const int l2 = brick_lane(si, sj, sk);
                 f[q] = P::ld(F, uint32_t(nb + (26 - q) * kLanes + l2), q);
             }
         }
@@ -689,8 +690,9 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
             for (int q = 0; q < 27; ++q) {
                 const int cx = qx(q), cy = qy(q), cz = qz(q);
                 const int di = i + cx, dj = j + cy, dk = k + cz;
-                const int nb = pick_brick(t, cx, cy, cz, (di & ~3) != 0, (dj & ~3) != 0, (dk & ~3) != 0);
-                const int l2 = (di & 3) | ((dj & 3) << 2) | ((dk & 3) << 4);
+                const int nb = pick_brick(t, cx, cy, cz, (di & ~(kBX - 1)) != 0, (dj & ~(kBY - 1)) != 0,
+                                          (dk & ~(kBZ - 1)) != 0);
+                const int l2 = brick_lane(di, dj, dk);
                 // BC cells skip the writes that would leave the grid; the
                 // buffer resource drops them anyway, this keeps them off the bus
                 if ((wmask >> q) & 1u) P::st(F, uint32_t(nb + q * kLanes + l2), q, f[q]);
@@ -764,7 +766,7 @@ __device__ __forceinline__ void prepass_brick(const StepLaunch& L, const BrickRe
     int t[27];
 #pragma unroll
     for (int q = 0; q < 27; ++q) t[q] = nbr_c[q];
-    const int i = lane & 3, j = (lane >> 2) & 3, k = lane >> 4;
+    const int i = lane & (kBX - 1), j = (lane >> kSX) & (kBY - 1), k = lane >> (kSX + kSY);
     R f[27];
     if (parity == 0) {
         const uint32_t own = uint32_t(t[13]) + uint32_t(lane);
@@ -775,8 +777,9 @@ __device__ __forceinline__ void prepass_brick(const StepLaunch& L, const BrickRe
         for (int q = 0; q < 27; ++q) {
             const int cx = qx(q), cy = qy(q), cz = qz(q);
             const int si = i - cx, sj = j - cy, sk = k - cz;
-            const int nb = pick_brick(t, -cx, -cy, -cz, (si & ~3) != 0, (sj & ~3) != 0, (sk & ~3) != 0);
-            const int l2 = (si & 3) | ((sj & 3) << 2) | ((sk & 3) << 4);
+            const int nb = pick_brick(t, -cx, -cy, -cz, (si & ~(kBX - 1)) != 0, (sj & ~(kBY - 1)) != 0,
+                                      (sk & ~(kBZ - 1)) != 0);
+            const int l2 = brick_lane(si, sj, sk);
             f[q] = P::ld(F, uint32_t(nb + (26 - q) * kLanes + l2), q);
         }
     }
@@ -824,6 +827,9 @@ __global__ void __launch_bounds__(64) k_prepass(StepLaunch L)
     if (int(blockIdx.x) < L.n_list) prepass_brick<S>(L, L.list[blockIdx.x], int(threadIdx.x));
 }
 
+// a brick and its one-cell halo, staged in LDS by the strain and force kernels
+constexpr int kHX = kBX + 2, kHY = kBY + 2, kHZ = kBZ + 2, kHalo = kHX * kHY * kHZ;
+
 // body force of the fd-force mode (core.h:663-781) from the strain of the
 // pre-pass velocities: F_c = sum_a d_a S_{a c} with the one-sided / central /
 // zero rule of the fluid neighbours, nu from the cell's own strain in single
@@ -836,31 +842,31 @@ __global__ void __launch_bounds__(64) k_force(const CaseDev* Cp)
     const CaseDev& C = *Cp;
     const int b = int(blockIdx.x);
     const int lane = int(threadIdx.x);
-    __shared__ R ss[6][216];
-    __shared__ uint8_t sf[216];
+    __shared__ R ss[6][kHalo];
+    __shared__ uint8_t sf[kHalo];
     const R* so = static_cast<const R*>(C.strain);
-    for (int c = lane; c < 216; c += 64) {
-        const int hx = c % 6, hy = (c / 6) % 6, hz = c / 36;
+    for (int c = lane; c < kHalo; c += 64) {
+        const int hx = c % kHX, hy = (c / kHX) % kHY, hz = c / (kHX * kHY);
         const int gx = hx - 1, gy = hy - 1, gz = hz - 1;
-        const int ox = (gx < 0) ? -1 : (gx > 3 ? 1 : 0);
-        const int oy = (gy < 0) ? -1 : (gy > 3 ? 1 : 0);
-        const int oz = (gz < 0) ? -1 : (gz > 3 ? 1 : 0);
+        const int ox = (gx < 0) ? -1 : (gx > kBX - 1 ? 1 : 0);
+        const int oy = (gy < 0) ? -1 : (gy > kBY - 1 ? 1 : 0);
+        const int oz = (gz < 0) ? -1 : (gz > kBZ - 1 ? 1 : 0);
         const int nb = C.nbr[b * 27 + 13 + 9 * ox + 3 * oy + oz] / (27 * kLanes);
-        const int l = (gx & 3) | ((gy & 3) << 2) | ((gz & 3) << 4);
+        const int l = brick_lane(gx, gy, gz);
         const bool fl = C.label[nb * kLanes + l] == L_FLUID && nb < C.n_active;
         sf[c] = fl ? 1 : 0;
 #pragma unroll
         for (int m = 0; m < 6; ++m) ss[m][c] = fl ? so[(nb * 6 + m) * kLanes + l] : R(0);
     }
     __syncthreads();
-    const int i = lane & 3, j = (lane >> 2) & 3, k = lane >> 4;
-    const int c0 = (i + 1) + 6 * ((j + 1) + 6 * (k + 1));
+    const int i = lane & (kBX - 1), j = (lane >> kSX) & (kBY - 1), k = lane >> (kSX + kSY);
+    const int c0 = (i + 1) + kHX * ((j + 1) + kHY * (k + 1));
     R Fo[3] = {R(0), R(0), R(0)};
     if (sf[c0]) {
         // S components: 0 S11, 1 S12, 2 S13, 3 S22, 4 S32, 5 S33; rows of the
         // derivative along x, y, z
         const int rows[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-        const int st[3] = {1, 6, 36};
+        const int st[3] = {1, kHX, kHX * kHY};
         R Fc[3] = {R(0), R(0), R(0)};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -918,24 +924,24 @@ __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
 // (cudaLBMKernelStress, solver/lbm3d/core.h:407-576): per axis, central
 // difference when both face neighbours are fluid, one-sided when one is, zero
 // when none is; S = 0 on non-fluid cells.  The brick and its one-cell halo are
-// staged in LDS (6x6x6 cells x (fluid flag, u)).
+// staged in LDS (its (kBX+2) x (kBY+2) x (kBZ+2) cells x (fluid flag, u)).
 template <class R>
 __global__ void __launch_bounds__(64) k_strain(const CaseDev* Cp)
 {
     const CaseDev& C = *Cp;
     const int b = int(blockIdx.x);
     const int lane = int(threadIdx.x);
-    __shared__ R su[3][216];
-    __shared__ uint8_t sf[216];
+    __shared__ R su[3][kHalo];
+    __shared__ uint8_t sf[kHalo];
     const R* mo = static_cast<const R*>(C.macro);
-    for (int c = lane; c < 216; c += 64) {
-        const int hx = c % 6, hy = (c / 6) % 6, hz = c / 36;
+    for (int c = lane; c < kHalo; c += 64) {
+        const int hx = c % kHX, hy = (c / kHX) % kHY, hz = c / (kHX * kHY);
         const int gx = hx - 1, gy = hy - 1, gz = hz - 1;  // cell offset inside the brick (-1..4)
-        const int ox = (gx < 0) ? -1 : (gx > 3 ? 1 : 0);
-        const int oy = (gy < 0) ? -1 : (gy > 3 ? 1 : 0);
-        const int oz = (gz < 0) ? -1 : (gz > 3 ? 1 : 0);
+        const int ox = (gx < 0) ? -1 : (gx > kBX - 1 ? 1 : 0);
+        const int oy = (gy < 0) ? -1 : (gy > kBY - 1 ? 1 : 0);
+        const int oz = (gz < 0) ? -1 : (gz > kBZ - 1 ? 1 : 0);
         const int nb = C.nbr[b * 27 + 13 + 9 * ox + 3 * oy + oz] / (27 * kLanes);
-        const int l = (gx & 3) | ((gy & 3) << 2) | ((gz & 3) << 4);
+        const int l = brick_lane(gx, gy, gz);
         const bool fl = C.label[nb * kLanes + l] == L_FLUID;
         sf[c] = fl ? 1 : 0;
         R v0 = R(0), v1 = R(0), v2 = R(0);
@@ -947,11 +953,11 @@ __global__ void __launch_bounds__(64) k_strain(const CaseDev* Cp)
         su[0][c] = v0; su[1][c] = v1; su[2][c] = v2;
     }
     __syncthreads();
-    const int i = lane & 3, j = (lane >> 2) & 3, k = lane >> 4;
-    const int c0 = (i + 1) + 6 * ((j + 1) + 6 * (k + 1));
+    const int i = lane & (kBX - 1), j = (lane >> kSX) & (kBY - 1), k = lane >> (kSX + kSY);
+    const int c0 = (i + 1) + kHX * ((j + 1) + kHY * (k + 1));
     R S11 = 0, S12 = 0, S13 = 0, S22 = 0, S32 = 0, S33 = 0;
     if (sf[c0]) {
-        const int st[3] = {1, 6, 36};
+        const int st[3] = {1, kHX, kHX * kHY};
         R d[3][3];  // d[axis][component]
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
