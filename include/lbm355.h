@@ -132,7 +132,7 @@ typedef struct lbm355_case_info {
     int64_t area_ivc, area_svc, area_out5, area_out6;  /* open BC-plane cells       */
     int64_t input_label3, input_label4, input_label5, input_label6; /* VTK counts   */
     int64_t active_cells;   /* non-wall cells updated per step                     */
-    int64_t bricks_active;  /* 4x4x4 bricks holding an active cell                 */
+    int64_t bricks_active;  /* 64-cell bricks holding an active cell               */
     int64_t bricks_stored;  /* bricks with population storage                      */
     int64_t device_bytes;   /* device memory held by the case                      */
     int64_t steps;          /* time steps done                                     */
@@ -186,7 +186,7 @@ int lbm355_case_set_inflow(lbm355_case* c, double ivc_flux, double svc_flux);
    case's stream; ends with the statistics and macro fields of the last step */
 int lbm355_case_step(lbm355_case* c, int64_t n);
 int lbm355_case_sync(lbm355_case* c);
-/* launch shape of the case's fused step: 4x4x4 bricks (wavefronts) per
+/* launch shape of the case's fused step: 64-cell bricks (wavefronts) per
    workgroup, 1, 2, 4 or 8 (default 4).  No counterpart in the reference, whose
    launch shape is the y-block size (--block-size, sim_tcpc.cu:36). */
 int lbm355_case_set_shape(lbm355_case* c, int bricks_per_group);
@@ -276,7 +276,7 @@ typedef struct lbm355_batch lbm355_batch;
 int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batch** out);
 int lbm355_batch_step(lbm355_batch* b, int64_t n);
 int lbm355_batch_destroy(lbm355_batch* b);
-/* launch shape: 4x4x4 bricks (wavefronts) per workgroup, 1, 2, 4 or 8 */
+/* launch shape: 64-cell bricks (wavefronts) per workgroup, 1, 2, 4 or 8 */
 int lbm355_batch_set_shape(lbm355_batch* b, int bricks_per_group);
 int lbm355_batch_set_occupancy(lbm355_batch* b, int waves_per_cu);
 /* members are stepped in groups of consecutive cases holding at most `bytes`

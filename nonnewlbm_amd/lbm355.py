@@ -79,7 +79,7 @@ class Case:
         check(self.lib.lbm355_case_sync(self.h), "sync")
 
     def set_shape(self, bricks_per_group: int) -> None:
-        """4x4x4 bricks per workgroup of the fused step (1, 2, 4 or 8)."""
+        """64-cell bricks (wavefronts) per workgroup of the fused step (1, 2, 4 or 8)."""
         check(self.lib.lbm355_case_set_shape(self.h, int(bricks_per_group)), "case_set_shape")
 
     def set_nontemporal(self, mode: int) -> None:
