@@ -95,3 +95,39 @@ def test_flows_known_velocity(tmp_path):
     assert row["lpa_split_percent"] == pytest.approx(100 * 0.64 / 1.2)
     assert row["rpa_split_percent"] + row["lpa_split_percent"] == pytest.approx(100.0)
     assert flows.parse_label_map("3:IVC,7:OUT") == {3: "IVC", 7: "OUT"}
+
+
+def test_run_all_dry_run_and_helpers(capsys):
+    """run_all prints the run_case command of every (regime, material) pair in
+    the configured order; the step count follows the sim_tcpc loop; the
+    device-flux summary row equals the flows module's row for the same plane
+    velocities."""
+    from nonnewlbm_amd import run_all, run_case
+    assert run_all.main(["--dry-run"]) == 0
+    out = capsys.readouterr().out.strip().splitlines()
+    pairs = run_all.case_matrix(None, None)
+    assert len(out) == len(pairs) == 15
+    root = run_case.repo_root()
+    for line, (regime, material) in zip(out, pairs):
+        assert line.split() == run_case.build_command(root, regime, material, f"{regime}_{material}")
+    # the first k with k dt > T
+    assert run_all.steps_for(1.0, 0.25) == 5 and run_all.steps_for(1.0, 0.3) == 4
+    # plane fluxes in lattice units against the VTK-based flows of the same cells
+    X, Y, Z = 20, 16, 12
+    dl, dt = 1e-3, 2e-3
+    wall = np.ones((X, Y, Z))
+    vel = np.zeros((X, Y, Z, 3))
+    wall[0, 5:8, 4:6] = 3
+    vel[0, 5:8, 4:6, 0] = 0.2
+    wall[X - 1, 9:11, 4:6] = 4
+    vel[X - 1, 9:11, 4:6, 0] = -0.1
+    wall[8:10, 0, 4:6] = 5
+    vel[8:10, 0, 4:6, 1] = -0.14
+    wall[8:10, Y - 1, 4:6] = 6
+    vel[8:10, Y - 1, 4:6, 1] = 0.16
+    ref = flows.compute_arrays(wall, vel, dl)
+    lat = dt / dl   # physical velocity -> lattice velocity
+    flux = [6 * 0.2 * lat, 4 * 0.1 * lat, 4 * 0.14 * lat, 4 * 0.16 * lat]
+    row = run_all.flows_row(flux, {"area_ivc": 6, "area_svc": 4, "area_out5": 4, "area_out6": 4}, dl, dt)
+    for k, v in row.items():
+        assert v == pytest.approx(ref[k], rel=1e-12, abs=1e-15), k
