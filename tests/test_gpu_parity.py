@@ -92,8 +92,10 @@ def test_tcpc_models(gpu_lib, collision, storage):
 
 @pytest.mark.parametrize("dims", [(65, 49, 33), (66, 50, 34), (67, 51, 35)])
 def test_brick_origin_shift(gpu_lib, dims):
-    """Grids whose last plane starts a new 4-cell brick get a shifted brick
-    origin; the BC partner reads must still match the oracle exactly."""
+    """Grids whose last x or y plane would start a new brick (X - 1 and Y - 1
+    multiples of the brick edge, as 65 x 49 for 8- and 4-cell bricks) get a
+    shifted brick origin; the BC partner reads must still match the oracle
+    exactly."""
     from nonnewlbm_amd.geometry import tcpc_mask
     g = tcpc_mask(0.25, dims=dims)
     kw = dict(storage=_abi.F64, collision=_abi.CUM, nu_ref=0.02, profile=_abi.PROFILE_PLUG,
