@@ -524,6 +524,20 @@ def main() -> int:
     alg_bytes = active * 2 * 27 * sbytes
     achieved = alg_bytes / (step_ms * 1e-3) / 1e9
     flux = case.flux(mean=False)
+    # host handover (outputs only; a step moves no host data): one dense
+    # double field read back, and the step rate when rho and u are read back
+    # every 100 steps
+    ncell = int(np.prod(g.dims))
+    t0 = time.perf_counter()
+    for _ in range(5):
+        case.field("vx")
+    readback_gbs = 5 * ncell * 8 / (time.perf_counter() - t0) / 1e9
+    t0 = time.perf_counter()
+    for _ in range(10):
+        case.step(100)
+        for name in ("rho", "vx", "vy", "vz"):
+            case.field(name)
+    mlups_readback = active * 1000 / (time.perf_counter() - t0) / 1e6
     copy_gbs = copy_bandwidth(1 << 30, 10)
     print(f"# k_step {step_ms * 1e3:.2f} us per launch back to back -> {achieved:.0f} GB/s algorithmic "
           f"({kern_ms * 1e3:.2f} us with an event pair per launch); copy {copy_gbs:.0f} GB/s; finite={finite}",
@@ -570,6 +584,8 @@ def main() -> int:
             "note": "working set below the 256 MB Infinity Cache: this fraction is cache-assisted",
         },
         "all_sites_mlups": round(int(np.prod(g.dims)) * args.steps / wall / 1e6, 1),
+        "host_handover": {"field_readback_gbs": round(readback_gbs, 2),
+                          "fluid_mlups_reading_rho_u_every_100_steps": round(mlups_readback, 1)},
         "step_us": round(step_ms * 1e3, 3),
         "copy_bandwidth_gbs": round(copy_gbs, 1),
         "results_finite": finite,
