@@ -99,3 +99,18 @@ def test_zero_area_terminates():
     lab, info = lbm355.labels(mask.transpose(2, 1, 0).ravel(), dims, storage=_abi.F64, nu_ref=0.05)
     # the whole y=0 row x = 0..15, z = 4,5 (the y planes overwrite the x planes)
     assert info["terminate"] == 0 and info["area_out5"] == 16 * 2
+
+
+def test_case_above_4gib_is_refused_before_the_device():
+    """Buffer-resource offsets are 32-bit: a case whose population buffer
+    would reach 4 GiB (an all-fluid 288x288x256 grid at FP64: 21.2 M cells,
+    about 4.6 GB) is refused at setup, before any device call."""
+    lib = _abi.load()
+    dims = (288, 288, 256)
+    geom = np.zeros(int(np.prod(dims)), dtype=np.int32)
+    desc, keep = _abi.make_desc(geom, dims, storage=_abi.F64, collision=_abi.CUM, nu_ref=0.02, dl=1e-3, dt=1e-4)
+    h = C.c_void_p()
+    rc = lib.lbm355_case_create(C.byref(desc), C.byref(h))
+    assert rc == _abi.EINVAL and not h.value
+    assert "4 GiB" in lib.lbm355_last_error().decode()
+    del keep
