@@ -208,7 +208,9 @@ int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes)
 /* populations after the last step in the reference's A/B sense: out[q][cell] =
    post-collision f_q of each active cell (df_out after SimUpdate); wall cells
    next to the flow carry the populations in flight through full-way
-   bounce-back, other wall cells are 0; bytes >= 27*nx*ny*nz*8 */
+   bounce-back, other wall cells are 0.  After an odd step the outward
+   directions of BC-plane cells read 0: they would leave the grid, nobody reads
+   them, and the step does not store them.  bytes >= 27*nx*ny*nz*8 */
 int lbm355_case_get_populations(lbm355_case* c, double* host, size_t bytes);
 
 /* finite-difference strain S of the current velocity field with the reference
