@@ -284,3 +284,26 @@ def test_near_4gib_case_matches_oracle(gpu_lib):
     assert worst < 1e-12, worst
     assert stored / total > 0.95
     assert float(np.max(np.abs(pg[13][act] - 8.0 / 27.0))) > 1e-6   # the inflow has moved the state
+
+
+@pytest.mark.parametrize("storage", [_abi.F64, _abi.F32])
+def test_inflow_hook_mid_run(gpu_lib, storage):
+    """updateKernelVelocities (sim_tcpc.cu:657-663) between steps: the caval
+    fluxes change after 7 steps (odd, so the next step is even) and again
+    after 12; the GPU follows the oracle through both changes."""
+    g, kw = tcpc_case(0.25, storage=storage)
+    gpu, cpu = _pair(g, kw)
+    for n, scale in ((7, 1.7), (5, 0.4), (9, None)):
+        gpu.step(n)
+        cpu.step(n)
+        if scale is not None:
+            a, b = kw["ivc_flux"] * scale, kw["svc_flux"] * (2.0 - scale)
+            gpu.set_inflow(a, b)
+            cpu.set_inflow(a, b)
+    act = _active(cpu)
+    tol = 1e-12 if storage == _abi.F64 else 1e-3   # FP32: hardware rcp/sqrt, as in test_tcpc_models
+    for name in ("rho", "vx", "vy", "vz"):
+        assert rel_l2(gpu.field(name), cpu.field(name), act) < tol, name
+    fl = gpu.flux(mean=False)
+    # plug inflow: the IVC plane carries exactly the prescribed lattice flux
+    assert fl[0] == pytest.approx(kw["ivc_flux"] * 0.4, rel=1e-5)
