@@ -915,6 +915,24 @@ int lbm355_case_sync(lbm355_case* c)
     return LBM355_OK;
 }
 
+// HIP events and a stream released on every return path
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    explicit EventSet(size_t n) : ev(n, nullptr) {}
+    ~EventSet()
+    {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+struct StreamHolder {
+    hipStream_t s = nullptr;
+    ~StreamHolder()
+    {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
 int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_step_ms, double* avg_kernel_ms)
 {
     if (!c || n <= 0) return fail(LBM355_EINVAL, "bad argument");
@@ -923,7 +941,8 @@ int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_step_ms, doub
     if ((rc = set_case_u64(*c, &c->host.macro_step, uint64_t(c->steps + n - 1)))) return rc;
     StepLaunch L = single_launch(*c);
     const int64_t nk = std::max<int64_t>(1, std::min<int64_t>(n2, 512));
-    std::vector<hipEvent_t> ev(size_t(2 * nk + 2));
+    EventSet es(size_t(2 * nk + 2));
+    std::vector<hipEvent_t>& ev = es.ev;
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipEventRecord(ev[2 * nk], c->stream));
     for (int64_t s = 0; s < n1; ++s) {
@@ -946,7 +965,6 @@ int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_step_ms, doub
     }
     float whole = 0;
     HIP_TRY(hipEventElapsedTime(&whole, ev[2 * nk], ev[2 * nk + 1]));
-    for (auto& e : ev) (void)hipEventDestroy(e);
     if (avg_kernel_ms) *avg_kernel_ms = n2 > 0 ? ksum / double(std::min(nk, n2)) : double(whole) / double(n1);
     if (avg_step_ms) *avg_step_ms = double(whole) / double(n1);
     if (c->d.stats_every > 0) {
@@ -965,11 +983,13 @@ int lbm355_copy_bandwidth(int64_t bytes, int iters, double* gbps)
     bytes &= ~int64_t(15);
     DevBuf a, b;
     if (a.alloc(size_t(bytes)) || b.alloc(size_t(bytes))) return fail(LBM355_ENOMEM, "copy buffers");
-    hipStream_t st;
-    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    StreamHolder sh;
+    HIP_TRY(hipStreamCreateWithFlags(&sh.s, hipStreamNonBlocking));
+    hipStream_t st = sh.s;
+    EventSet es(2);
+    HIP_TRY(hipEventCreate(&es.ev[0]));
+    HIP_TRY(hipEventCreate(&es.ev[1]));
+    hipEvent_t e0 = es.ev[0], e1 = es.ev[1];
     LAUNCH_TRY(launch_zero(a.p, bytes, st));
     LAUNCH_TRY(launch_copy(b.p, a.p, bytes, st));  // warm-up
     double best = 0.0;
@@ -982,9 +1002,6 @@ int lbm355_copy_bandwidth(int64_t bytes, int iters, double* gbps)
         HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
         if (ms > 0) best = std::max(best, 2.0 * double(bytes) / (double(ms) * 1e6));
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(st);
     *gbps = best;
     return LBM355_OK;
 }
