@@ -28,6 +28,7 @@ index, a transposed operand) fails at least one of these.
 from __future__ import annotations
 
 import itertools
+import zlib
 
 import numpy as np
 import pytest
@@ -41,6 +42,17 @@ CV = np.stack([Q // 9 - 1, (Q // 3) % 3 - 1, Q % 3 - 1], axis=1)   # defs.h:215-
 OPP = 26 - Q
 WQ = np.array([{0: 8 / 27, 1: 2 / 27, 2: 1 / 54, 3: 1 / 216}[int((c * c).sum())] for c in CV])
 RNG = np.random.default_rng(20261203)
+# FP64 rounding of O(1) quantities that go through sums of 27 terms (27 ulp of
+# 0.3, twice): the floor of the moment-level comparisons below
+SUM27 = 4e-15
+
+
+@pytest.fixture(autouse=True)
+def _own_stream(request):
+    """Each test draws from its own seeded stream, whatever ran before it in
+    the same process (the helpers below read the module-level RNG)."""
+    global RNG
+    RNG = np.random.default_rng([20261203, zlib.crc32(request.node.name.encode())])
 
 
 def central_matrix(u) -> np.ndarray:
@@ -161,11 +173,11 @@ def test_cumulant_equilibrium_special_cases():
         u = 0.08 * (2 * RNG.random(3) - 1)
         feq = feq_product(rho, u)
         for nu in (0.004, 1.0 / 6.0, 0.3):
-            np.testing.assert_allclose(O.collide(_abi.CUM, feq, rho, u, nu=nu), feq, rtol=0, atol=1e-15)
+            np.testing.assert_allclose(O.collide(_abi.CUM, feq, rho, u, nu=nu), feq, rtol=0, atol=SUM27)
         f = random_state()
         r, *v = O.density_velocity(f)
         np.testing.assert_allclose(O.collide(_abi.CUM, f, r, v, nu=1.0 / 6.0), feq_product(r, v), rtol=0,
-                                   atol=1e-15)
+                                   atol=SUM27)
 
 
 def _orth_to_conserved(d):
@@ -233,18 +245,21 @@ def test_kbc_n4():
         # the shear part carries the whole second moment and the seven third
         # moments, which relax by 1 - 2 beta whatever the entropic gamma
         np.testing.assert_allclose(moments2(out) - moments2(feq), (1 - 2 * beta) * (moments2(f) - moments2(feq)),
-                                   rtol=0, atol=1e-15)
+                                   rtol=0, atol=SUM27)
         np.testing.assert_allclose(third7(out) - third7(feq), (1 - 2 * beta) * (third7(f) - third7(feq)),
-                                   rtol=0, atol=1e-15)
+                                   rtol=0, atol=SUM27)
         # entropic gamma (quadratic entropy): the post-collision deviation is
         # orthogonal, in the 1/f_eq product, to r = out - feq - (1-2b)(f - feq)
         r = out - feq - (1 - 2 * beta) * (f - feq)
         dev = out - feq
+        # the floor is FP64 rounding of out (about 64 ulp in the same norm),
+        # for draws where r itself is only a few ulp of out
         num = abs(np.sum(dev * r / feq))
-        assert num < 1e-12 * np.sqrt(np.sum(dev * dev / feq) * np.sum(r * r / feq)) + 1e-30
+        nd, nr, no = (np.sqrt(np.sum(v * v / feq)) for v in (dev, r, out))
+        assert num < 1e-12 * nd * nr + 64 * np.finfo(float).eps * nd * no
         # nu = 1/6 (beta = 1/2): equilibrium; f_eq is a fixed point
-        np.testing.assert_allclose(O.collide(_abi.KBC, f, rho, u, nu=1.0 / 6.0), feq, rtol=0, atol=1e-15)
-        np.testing.assert_allclose(O.collide(_abi.KBC, feq, rho, u, nu=nu), feq, rtol=0, atol=1e-15)
+        np.testing.assert_allclose(O.collide(_abi.KBC, f, rho, u, nu=1.0 / 6.0), feq, rtol=0, atol=SUM27)
+        np.testing.assert_allclose(O.collide(_abi.KBC, feq, rho, u, nu=nu), feq, rtol=0, atol=SUM27)
 
 
 # ---------------------------------------------------------------------------
