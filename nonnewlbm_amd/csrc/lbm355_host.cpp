@@ -204,7 +204,7 @@ struct lbm355_case {
     int64_t macro_valid_step = -1;      // step index whose rho,u are in the macro buffer
     int64_t stats_steps = 0;            // samples accumulated since the last reset
     int64_t stats_first = 0;            // step index of the epoch (last reset or restart)
-    int64_t stats_block0 = 0;           // stats_steps at the epoch: FP32 blocks count from there
+    int64_t stats_block0 = 0;           // stats_steps at the epoch: sample groups count from there
     bool strain_valid = false;
 
     hipStream_t stream = nullptr;
@@ -300,6 +300,7 @@ int64_t footprint_bytes(const lbm355_case_desc& d, int64_t n_active, int64_t n_s
     b += n_active * 6 * kLanes * cb;                      // strain
     if (d.cy_enabled && d.cy_model == LBM355_CY_FD_FORCE) b += n_active * 3 * kLanes * cb;  // body force
     if (d.stats_every > 0) b += n_active * kStatFields * kLanes * 8;
+    if (d.stats_every > 0 && d.storage != LBM355_F64) b += (kStatBlock - 1) * n_active * kRingFields * kLanes * 4;
     b += n_bc * (4 * cb + 2 * kQ * 4 + 4 + 4 + 1 + 8);   // bc tables
     b += n_active * 8;                                    // brick list
     return b;
@@ -529,7 +530,10 @@ int setup_device(lbm355_case& c)
     const bool fd_force = d.cy_enabled && d.cy_model == LBM355_CY_FD_FORCE;
     if (fd_force && (rc = alloc_zero(c, c.force, size_t(std::max(na, 1)) * 3 * kLanes * cb))) return rc;
     if (d.stats_every > 0 && (rc = alloc_zero(c, c.stats, size_t(std::max(na, 1)) * kStatFields * kLanes * 8))) return rc;
-    if (d.stats_every > 0 && (rc = alloc_zero(c, c.stats_part, size_t(std::max(na, 1)) * kStatFields * kLanes * 4)))
+    // ring of FP32 rho u samples, one slot per sample of the open group
+    // (FP64 storage adds every sample to the FP64 sums directly)
+    const bool ring = d.stats_every > 0 && d.storage != LBM355_F64;
+    if (ring && (rc = alloc_zero(c, c.stats_part, size_t(kStatBlock - 1) * std::max(na, 1) * kRingFields * kLanes * 4)))
         return rc;
     if ((rc = alloc_zero(c, c.base, 8))) return rc;
     if ((rc = alloc_zero(c, c.flux, 4 * 8))) return rc;
@@ -541,7 +545,7 @@ int setup_device(lbm355_case& c)
     std::memset(&h, 0, sizeof(h));
     h.f = c.f.p;
     h.stats = static_cast<double*>(d.stats_every > 0 ? c.stats.p : nullptr);
-    h.stats_part = static_cast<float*>(d.stats_every > 0 ? c.stats_part.p : nullptr);
+    h.stats_part = static_cast<float*>(ring ? c.stats_part.p : nullptr);
     h.stats_epoch = 0;
     h.macro = c.macro.p;
     h.strain = c.strain.p;
@@ -664,6 +668,12 @@ StepLaunch single_launch(lbm355_case& c)
     L.nontemporal = c.nontemporal >= 0 ? c.nontemporal : (case_device_bytes(c) > kNontemporalBytes ? 1 : 0);
     L.lds_bytes = occupancy_lds(c.waves_cap, c.bricks_per_group);
     return L;
+}
+
+// slot sl of the statistics ring: [active brick][3][64] FP32 rho u samples
+const float* ring_slot(const lbm355_case& c, int64_t sl)
+{
+    return static_cast<const float*>(c.stats_part.p) + sl * int64_t(std::max(c.n_active, 1)) * kRingFields * kLanes;
 }
 
 // copy a per-active-brick device array [brick][nf][64] of type R to dense
@@ -1039,14 +1049,15 @@ int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes)
     }
     if (field >= LBM355_FIELD_SUM_RUX && field <= LBM355_FIELD_SUM_RRUZZ) {
         if (c->d.stats_every <= 0) { std::fill(host, host + N, 0.0); return LBM355_OK; }
-        if ((rc = gather_dense<double>(*c, c->stats.p, kStatFields, field - LBM355_FIELD_SUM_RUX, 0.0, host))) return rc;
-        if (!f64 && (c->stats_steps - c->stats_block0) % kStatBlock != 0) {
-            // samples of the open FP32 block are not in the FP64 sums yet
-            std::vector<double> part(static_cast<size_t>(N));
-            if ((rc = gather_dense<float>(*c, c->stats_part.p, kStatFields, field - LBM355_FIELD_SUM_RUX, 0.0,
-                                          part.data())))
+        const int sf = field - LBM355_FIELD_SUM_RUX;
+        if ((rc = gather_dense<double>(*c, c->stats.p, kStatFields, sf, 0.0, host))) return rc;
+        // samples of the open group are in the ring, not in the FP64 sums yet
+        const int64_t open = f64 ? 0 : (c->stats_steps - c->stats_block0) % kStatBlock;
+        std::vector<double> part(open > 0 ? static_cast<size_t>(N) : 0);
+        for (int64_t sl = 0; sl < open; ++sl) {
+            if ((rc = gather_dense<float>(*c, ring_slot(*c, sl), kRingFields, sf % kRingFields, 0.0, part.data())))
                 return rc;
-            for (int64_t i = 0; i < N; ++i) host[i] += part[size_t(i)];
+            for (int64_t i = 0; i < N; ++i) host[i] += sf < kRingFields ? part[size_t(i)] : part[size_t(i)] * part[size_t(i)];
         }
         return LBM355_OK;
     }
@@ -1146,11 +1157,15 @@ int lbm355_case_flux(lbm355_case* c, int mean, double out[4])
     std::vector<double> h(size_t(std::max(na, 1)) * kStatFields * kLanes);
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (na > 0) HIP_TRY(hipMemcpy(h.data(), c->stats.p, h.size() * 8, hipMemcpyDeviceToHost));
-    if (c->storage != LBM355_F64 && (c->stats_steps - c->stats_block0) % kStatBlock != 0) {
-        // the open FP32 block of samples
-        std::vector<float> part(h.size());
-        if (na > 0) HIP_TRY(hipMemcpy(part.data(), c->stats_part.p, part.size() * 4, hipMemcpyDeviceToHost));
-        for (size_t i = 0; i < h.size(); ++i) h[i] += double(part[i]);
+    // samples of the open group, from the ring (only the first moments are used here)
+    const int64_t open = c->storage == LBM355_F64 ? 0 : (c->stats_steps - c->stats_block0) % kStatBlock;
+    std::vector<float> part(size_t(std::max(na, 1)) * kRingFields * kLanes);
+    for (int64_t sl = 0; sl < open; ++sl) {
+        if (na > 0) HIP_TRY(hipMemcpy(part.data(), ring_slot(*c, sl), part.size() * 4, hipMemcpyDeviceToHost));
+        for (int b = 0; b < na; ++b)
+            for (int a = 0; a < kRingFields; ++a)
+                for (int l = 0; l < kLanes; ++l)
+                    h[(size_t(b) * kStatFields + a) * kLanes + l] += double(part[(size_t(b) * kRingFields + a) * kLanes + l]);
     }
     const double diff = std::max<double>(1.0, double(c->stats_steps));
     for (int p = 0; p < 4; ++p) out[p] = 0.0;
@@ -1468,7 +1483,7 @@ int lbm355_case_load_state(lbm355_case* c, const char* dir, lbm355_state_desc* s
         }
         if (rc) return rc;
     }
-    // statistics: samples since the last reset; FP32 blocks restart here
+    // statistics: samples since the last reset; sample groups restart here
     const int64_t k = c->d.stats_every > 0 ? c->d.stats_every : 1;
     c->stats_steps = samples >= 0 ? samples : int64_t(it) / k - int64_t(prev) / k;
     c->stats_block0 = c->stats_steps;

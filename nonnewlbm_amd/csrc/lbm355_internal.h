@@ -44,7 +44,8 @@ __host__ __device__ constexpr int brick_lane(int i, int j, int k)
 constexpr int kQ = 27;               // D3Q27
 constexpr int kStatFields = 6;       // sum rho u (3), sum (rho u)^2 (3)
 constexpr int kMacroFields = 4;      // rho, ux, uy, uz
-constexpr int kStatBlock = 16;       // FP32 partial samples per FP64 flush
+constexpr int kStatBlock = 8;        // samples per addition to the FP64 sums (non-FP64 storage)
+constexpr int kRingFields = 3;       // rho u of one sample in a statistics ring slot
 
 // cell labels (the reference map values, sim_tcpc.cu:194-203, plus 7/8 for
 // outflow planes on x faces) and 255 = outside the grid
@@ -62,7 +63,7 @@ __host__ __device__ constexpr bool is_inflow_label(int l) { return l == L_IN_XM 
 struct CaseDev {
     void* f;                  // populations, storage type of the launch
     double* stats;            // [active brick][6][64] FP64 statistics sums, or null
-    float* stats_part;        // [active brick][6][64] FP32 partial sums of the open block
+    float* stats_part;        // [kStatBlock - 1][active brick][3][64] FP32 rho u samples of the open group
     void* macro;              // [active brick][4][64] rho,u (compute type), or null
     void* strain;             // [active brick][6][64] strain (compute type), or null
     void* force;              // [active brick][3][64] body force (fd-force), or null

@@ -620,15 +620,16 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
     // per-step output decisions, wave-uniform (32-bit step arithmetic)
     const uint32_t next = uint32_t(step) + 1u;
     // statistics are sampled at the steps whose index + 1 is a multiple of
-    // stats_every (as MacroTcpc every step when it is 1); samples are summed
-    // in FP32 blocks of kStatBlock counted from the epoch (the last reset or
-    // restart) and each block is added to the FP64 sums
+    // stats_every (as MacroTcpc every step when it is 1).  Samples are counted
+    // in groups of kStatBlock from the epoch (the last reset or restart): the
+    // first kStatBlock - 1 of a group store their rho u in a ring of FP32
+    // slots, the last adds the whole group to the FP64 sums
     const uint32_t kev = uint32_t(C.stats_every > 0 ? C.stats_every : 1);
     const uint32_t s32 = uint32_t(step), e32 = uint32_t(C.stats_epoch);
     const bool do_stats = C.stats && C.stats_every > 0 && (kev == 1u || (s32 + 1u) % kev == 0u);
     const uint32_t nsamp = kev == 1u ? s32 - e32 + 1u : (s32 + 1u) / kev - e32 / kev;  // samples since the epoch
     const uint32_t spos = (nsamp - 1u) % kStatBlock;
-    const rsrc_t PR = make_rsrc(C.stats_part, C.stats_bytes / 2);
+    const int64_t ring_slot_bytes = int64_t(C.n_active) * kRingFields * kLanes * 4;
     const bool do_macro = C.macro && (step == C.macro_step ||
                                       (C.macro_every > 0 && next % uint32_t(C.macro_every) == 0u));
     const rsrc_t SR = make_rsrc(C.stats, C.stats_bytes);
@@ -750,28 +751,39 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
 #pragma unroll
                 for (int a = 0; a < 6; ++a) st_f64(SR, (idx + uint32_t(a * kLanes)) * 8u, cur[a] + add[a]);
             } else {
-                const float rx = float(rho * ux), ry = float(rho * uy), rz = float(rho * uz);
-                const float add[6] = {rx, ry, rz, rx * rx, ry * ry, rz * rz};
-                if (spos == 0u) {
+                // 12 B per sample into ring slot spos; the group's last sample
+                // reads the kStatBlock - 1 slots and adds the group to the FP64
+                // sums (squares of the FP32 samples taken in FP64)
+                const float r[3] = {float(rho * ux), float(rho * uy), float(rho * uz)};
+                const uint32_t ridx = uint32_t(b * kRingFields) * kLanes + uint32_t(lane);
+                if (spos < uint32_t(kStatBlock - 1)) {
+                    const rsrc_t RS = make_rsrc(C.stats_part + int64_t(spos) * (ring_slot_bytes / 4), ring_slot_bytes);
 #pragma unroll
-                    for (int a = 0; a < 6; ++a) st_f32(PR, (idx + uint32_t(a * kLanes)) * 4u, add[a]);
-                } else if (spos < uint32_t(kStatBlock - 1)) {
-                    float cur[6];
-#pragma unroll
-                    for (int a = 0; a < 6; ++a) cur[a] = ld_f32(PR, (idx + uint32_t(a * kLanes)) * 4u);
-#pragma unroll
-                    for (int a = 0; a < 6; ++a) st_f32(PR, (idx + uint32_t(a * kLanes)) * 4u, cur[a] + add[a]);
+                    for (int a = 0; a < 3; ++a) st_f32(RS, (ridx + uint32_t(a * kLanes)) * 4u, r[a]);
                 } else {
-                    float part[6];
+                    float ring[kStatBlock - 1][3];
+#pragma unroll
+                    for (int sl = 0; sl < kStatBlock - 1; ++sl) {
+                        const rsrc_t RS = make_rsrc(C.stats_part + int64_t(sl) * (ring_slot_bytes / 4), ring_slot_bytes);
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) ring[sl][a] = ld_f32(RS, (ridx + uint32_t(a * kLanes)) * 4u);
+                    }
                     double cur[6];
 #pragma unroll
-                    for (int a = 0; a < 6; ++a) {
-                        part[a] = ld_f32(PR, (idx + uint32_t(a * kLanes)) * 4u);
-                        cur[a] = ld_f64(SR, (idx + uint32_t(a * kLanes)) * 8u);
-                    }
+                    for (int a = 0; a < 6; ++a) cur[a] = ld_f64(SR, (idx + uint32_t(a * kLanes)) * 8u);
 #pragma unroll
-                    for (int a = 0; a < 6; ++a)
-                        st_f64(SR, (idx + uint32_t(a * kLanes)) * 8u, cur[a] + double(part[a] + add[a]));
+                    for (int a = 0; a < 3; ++a) {
+                        double sum = 0.0, sq = 0.0;
+#pragma unroll
+                        for (int sl = 0; sl < kStatBlock - 1; ++sl) {
+                            const double v = double(ring[sl][a]);
+                            sum += v;
+                            sq += v * v;
+                        }
+                        const double v = double(r[a]);
+                        st_f64(SR, (idx + uint32_t(a * kLanes)) * 8u, cur[a] + (sum + v));
+                        st_f64(SR, (idx + uint32_t((a + 3) * kLanes)) * 8u, cur[a + 3] + (sq + v * v));
+                    }
                 }
             }
         }

@@ -114,32 +114,42 @@ def test_brick_origin_shift(gpu_lib, dims):
         assert rel_l2(gpu.field(name), cpu.field(name), act) < 1e-9, name
 
 
+@pytest.mark.parametrize("every", [1, 3])
 @pytest.mark.parametrize("storage", [_abi.F64, _abi.F32])
-def test_statistics_sums(gpu_lib, storage):
+def test_statistics_sums(gpu_lib, storage, every):
     """Mean/rms sums (MacroTcpc e_vm_*, sim_tcpc.cu:347-353) against the
-    oracle, over a length that leaves an open FP32 partial block, and after a
-    statistics reset."""
-    g, kw = channel_case(storage)
+    oracle: read while a group of samples is still open (37 steps), after a
+    call that closes it (40), and after a statistics reset, where the sums of
+    steps 40..59 are the oracle's sums at 60 minus those at 40."""
+    g, kw = channel_case(storage, stats_every=every)
     gpu = Case(g.flat(), g.dims, **kw)
     cpu = OracleCase(g.flat(), g.dims, **dict(kw, storage=_abi.F64))
-    gpu.step(37)
-    cpu.step(37)
     act = _active(cpu)
     tol = 1e-9 if storage == _abi.F64 else 2e-5
-    for name in ("sum_rux", "sum_ruy", "sum_ruz", "sum_rruxx", "sum_rruyy", "sum_rruzz"):
+    names = ("sum_rux", "sum_ruy", "sum_ruz", "sum_rruxx", "sum_rruyy", "sum_rruzz")
+    gpu.step(37)
+    cpu.step(37)
+    for name in names:
         assert rel_l2(gpu.field(name), cpu.field(name), act) < tol, name
     # instantaneous inlet flux = the prescribed profile sum; the mean flux is
     # the density-weighted mean momentum of the plane cells (mean_velocity)
     lab = gpu.field("label")
     inst = gpu.flux(mean=False)
     np.testing.assert_allclose(inst[0], cpu.field("vx")[lab == 3].sum(), rtol=1e-6)
+    n = 37 // every
     mean = gpu.flux(mean=True)
-    np.testing.assert_allclose(mean[0], gpu.field("sum_rux")[lab == 3].sum() / 37.0, rtol=1e-9)
-    np.testing.assert_allclose(mean[1], gpu.field("sum_rux")[lab == 7].sum() / 37.0, rtol=1e-9)
+    np.testing.assert_allclose(mean[0], gpu.field("sum_rux")[lab == 3].sum() / n, rtol=1e-9)
+    np.testing.assert_allclose(mean[1], gpu.field("sum_rux")[lab == 7].sum() / n, rtol=1e-9)
+    gpu.step(3)
+    cpu.step(3)
+    at40 = {name: cpu.field(name) for name in names}
+    for name in names:
+        assert rel_l2(gpu.field(name), at40[name], act) < tol, name
     gpu.reset_stats()
     gpu.step(20)
-    s = gpu.field("sum_rux")[act]
-    assert np.isfinite(s).all() and np.abs(s).max() > 0
+    cpu.step(20)
+    for name in names:
+        assert rel_l2(gpu.field(name), cpu.field(name) - at40[name], act) < tol, name
 
 
 @pytest.mark.parametrize("storage", [_abi.F64, _abi.F32])
