@@ -208,7 +208,7 @@ struct lbm355_case {
     bool strain_valid = false;
 
     hipStream_t stream = nullptr;
-    DevBuf f, stats, stats_part, macro, strain, force, nbr, label, bc_base, bc_cell_d, bc_read, bc_wmask, bc_factor, bc_label,
+    DevBuf f, stats, stats_part, macro, strain, force, nbr, label, bc_base, bc_cell_d, bc_read, bc_factor, bc_label,
         origin, base, devcase, list, flux;
     CaseDev host{};
     int bricks_per_group = 4;
@@ -293,15 +293,15 @@ int validate(const lbm355_case_desc* d)
 int64_t footprint_bytes(const lbm355_case_desc& d, int64_t n_active, int64_t n_stored, int64_t n_bc)
 {
     const int64_t sb = storage_bytes(d.storage), cb = compute_bytes(d.storage);
-    int64_t b = (n_stored + 1) * kQ * kLanes * sb;       // populations
-    b += (n_stored + 1) * kLanes;                         // labels
-    b += n_active * (kQ * 4 + 4 + 6);                     // nbr, bc_base, origin
-    b += n_active * kMacroFields * kLanes * cb;           // macro
-    b += n_active * 6 * kLanes * cb;                      // strain
+    int64_t b = (n_stored + kSpareBricks) * kQ * kLanes * sb;   // populations
+    b += (n_stored + kSpareBricks) * kLanes;                     // labels
+    b += n_active * (kQ * 4 + 4 + 6);                            // nbr, bc_base, origin
+    b += n_active * kMacroFields * kLanes * cb;                  // macro
+    b += n_active * 6 * kLanes * cb;                             // strain
     if (d.cy_enabled && d.cy_model == LBM355_CY_FD_FORCE) b += n_active * 3 * kLanes * cb;  // body force
     if (d.stats_every > 0) b += n_active * kStatFields * kLanes * 8;
     if (d.stats_every > 0 && d.storage != LBM355_F64) b += (kStatBlock - 1) * n_active * kRingFields * kLanes * 4;
-    b += n_bc * (4 * cb + 2 * kQ * 4 + 4 + 4 + 1 + 8);   // bc tables
+    b += n_bc * (4 * cb + 2 * kQ * 4 + 4 + 1 + 8);       // bc tables
     b += n_active * 8;                                    // brick list
     return b;
 }
@@ -366,7 +366,7 @@ int build_index(lbm355_case& c)
     std::sort(shell.begin(), shell.end());
     c.n_active = int32_t(act.size());
     c.n_stored = int32_t(act.size() + shell.size());
-    if (int64_t(c.n_stored + 1) * kQ * kLanes * int64_t(storage_bytes(d.storage)) >= (int64_t(1) << 32))
+    if (int64_t(c.n_stored + kSpareBricks) * kQ * kLanes * int64_t(storage_bytes(d.storage)) >= (int64_t(1) << 32))
         return fail(LBM355_EINVAL, "case too large: populations must stay below 4 GiB per case");
     c.brick_id.assign(NB, -1);
     c.brick_xyz.resize(size_t(c.n_stored) * 3);
@@ -406,8 +406,8 @@ int setup_device(lbm355_case& c)
     const int na = c.n_active, ns = c.n_stored;
     HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
 
-    // labels per stored brick (+ dummy = wall)
-    std::vector<uint8_t> lab(size_t(ns + 1) * kLanes, L_WALL);
+    // labels per stored brick (+ dummy and sink = wall)
+    std::vector<uint8_t> lab(size_t(ns + kSpareBricks) * kLanes, L_WALL);
     for (int b = 0; b < ns; ++b)
         for (int l = 0; l < kLanes; ++l) {
             int x, y, z;
@@ -415,8 +415,10 @@ int setup_device(lbm355_case& c)
             lab[size_t(b) * kLanes + l] =
                 (x >= 0 && y >= 0 && x < X && y < Y && z < Z) ? L.map[L.at(x, y, z)] : L_OUTSIDE;
         }
-    // neighbour bricks of the active bricks
-    std::vector<int32_t> nbr(size_t(std::max(na, 1)) * kQ, ns * kQ * kLanes);
+    // neighbour bricks of the active bricks; absent ones are the sink, which
+    // takes the odd step's writes that leave the grid
+    const int32_t sink = (ns + 1) * kQ * kLanes;
+    std::vector<int32_t> nbr(size_t(std::max(na, 1)) * kQ, sink);
     std::vector<int16_t> org(size_t(std::max(na, 1)) * 3, 0);
     for (int b = 0; b < na; ++b) {
         const int bx = c.brick_xyz[3 * b], by = c.brick_xyz[3 * b + 1], bz = c.brick_xyz[3 * b + 2];
@@ -427,12 +429,11 @@ int setup_device(lbm355_case& c)
             if (nx >= 0 && ny >= 0 && nz >= 0 && nx < c.nbx && ny < c.nby && nz < c.nbz)
                 s = c.brick_id[nx + int64_t(c.nbx) * (ny + int64_t(c.nby) * nz)];
             // element offset of the neighbour brick's population block
-            nbr[size_t(b) * kQ + o] = (s < 0 ? ns : s) * kQ * kLanes;
+            nbr[size_t(b) * kQ + o] = s < 0 ? sink : s * kQ * kLanes;
         }
     }
     // BC slots in (brick, lane) order
     std::vector<int32_t> bc_base(size_t(std::max(na, 1)), -1), bc_read;
-    std::vector<uint32_t> bc_wmask;
     std::vector<double> bc_factor;
     c.bc_cell.clear();
     c.bc_label_h.clear();
@@ -488,12 +489,6 @@ int setup_device(lbm355_case& c)
                 bc_read.push_back(c.slot_offset(src[q][0] + qx(q), src[q][1] + qy(q), src[q][2] + qz(q), q));
             // parity 1: slot opp(q) of C_q
             for (int q = 0; q < kQ; ++q) bc_read.push_back(c.slot_offset(src[q][0], src[q][1], src[q][2], 26 - q));
-            uint32_t wm = 0;
-            for (int q = 0; q < kQ; ++q) {
-                const int tx = x + qx(q), ty = y + qy(q), tz = z + qz(q);
-                if (tx >= 0 && ty >= 0 && tz >= 0 && tx < X && ty < Y && tz < Z) wm |= 1u << q;
-            }
-            bc_wmask.push_back(wm);
             double fac = 1.0;
             if (is_inflow_label(lb)) {
                 const int side = lb == L_IN_XM ? 0 : 1;
@@ -512,15 +507,14 @@ int setup_device(lbm355_case& c)
     c.n_bc = int32_t(c.bc_cell.size());
 
     const size_t sb = storage_bytes(d.storage), cb = compute_bytes(d.storage);
-    if (c.f.alloc(size_t(ns + 1) * kQ * kLanes * sb) != 0) return fail(LBM355_ENOMEM, "population buffer");
-    LAUNCH_TRY(launch_fill_populations(d.storage, c.f.p, ns + 1, c.stream));
+    if (c.f.alloc(size_t(ns + kSpareBricks) * kQ * kLanes * sb) != 0) return fail(LBM355_ENOMEM, "population buffer");
+    LAUNCH_TRY(launch_fill_populations(d.storage, c.f.p, ns + kSpareBricks, c.stream));
     int rc;
     if ((rc = upload(c.label, lab))) return rc;
     if ((rc = upload(c.nbr, nbr))) return rc;
     if ((rc = upload(c.origin, org))) return rc;
     if ((rc = upload(c.bc_base, bc_base))) return rc;
     if ((rc = upload(c.bc_read, bc_read))) return rc;
-    if ((rc = upload(c.bc_wmask, bc_wmask))) return rc;
     if ((rc = upload(c.bc_factor, bc_factor))) return rc;
     if ((rc = upload(c.bc_label, c.bc_label_h))) return rc;
     if ((rc = upload(c.bc_cell_d, c.bc_cell))) return rc;
@@ -555,7 +549,6 @@ int setup_device(lbm355_case& c)
     h.bc_base = static_cast<const int32_t*>(c.bc_base.p);
     h.bc_cell = static_cast<const int32_t*>(c.bc_cell_d.p);
     h.bc_read = static_cast<const int32_t*>(c.bc_read.p);
-    h.bc_wmask = static_cast<const uint32_t*>(c.bc_wmask.p);
     h.bc_factor = static_cast<const double*>(c.bc_factor.p);
     h.plug = d.profile == LBM355_PROFILE_PARABOLIC ? 0 : 1;
     h.bc_label = static_cast<const uint8_t*>(c.bc_label.p);
@@ -569,6 +562,19 @@ int setup_device(lbm355_case& c)
     h.nx = X; h.ny = Y; h.nz = Z;
     h.nu_ref = d.nu_ref;
     h.cy_nu0 = d.cy_nu0; h.cy_lambda = d.cy_lambda; h.cy_a = d.cy_a; h.cy_n = d.cy_n;
+    {
+        const double tr = 3.0 * d.nu_ref + 0.5, t0 = 3.0 * d.cy_nu0 + 0.5;
+        h.cy_e = (d.cy_n - 1.0) / d.cy_a;
+        h.cy_log2_lambda = std::log2(d.cy_lambda);
+        h.cy_log2_tau_max = std::log2(std::max(tr, t0));
+        h.cy_log2_tau_min = std::log2(std::min(tr, t0));
+        h.nu_ref_f = float(d.nu_ref);
+        h.cy_nu0_f = float(d.cy_nu0); h.cy_a_f = float(d.cy_a); h.cy_n_f = float(d.cy_n);
+        h.cy_e_f = float(h.cy_e);
+        h.cy_log2_lambda_f = float(h.cy_log2_lambda);
+        h.cy_log2_tau_max_f = float(h.cy_log2_tau_max);
+        h.cy_log2_tau_min_f = float(h.cy_log2_tau_min);
+    }
     h.cy_mode = d.cy_enabled ? d.cy_model : 0;
     h.inflow_xm = c.inflow[0];
     h.inflow_xp = c.inflow[1];
@@ -611,7 +617,7 @@ int64_t case_device_bytes(const lbm355_case& c)
 {
     int64_t bytes = 0;
     for (const DevBuf* b : {&c.f, &c.stats, &c.stats_part, &c.macro, &c.strain, &c.force, &c.nbr, &c.label, &c.bc_base,
-                            &c.bc_cell_d, &c.bc_read, &c.bc_wmask, &c.bc_factor, &c.bc_label, &c.origin, &c.base,
+                            &c.bc_cell_d, &c.bc_read, &c.bc_factor, &c.bc_label, &c.origin, &c.base,
                             &c.devcase, &c.list, &c.flux})
         bytes += int64_t(b->bytes);
     return bytes;

@@ -7,8 +7,12 @@
 //   Bricks are stored only where the AA update touches memory (the non-wall
 //   cells and their 26-neighbourhood); active bricks (holding at least one
 //   non-wall cell) come first, in Morton order of the brick coordinates, then
-//   the storage-only shell bricks, then one "dummy" brick that stands in for
-//   every brick that is not stored.
+//   the storage-only shell bricks, then two bricks that stand in for every
+//   brick that is not stored: the "dummy" (index n_stored), read by boundary
+//   partner lookups of absent cells and never written, and the "sink"
+//   (n_stored + 1), the target of the neighbour table for absent bricks.  The
+//   odd step's writes from boundary-plane cells that leave the grid land in
+//   the sink or in outside lanes of stored bricks, which nothing reads.
 //   populations : f[brick][q][lane], lane = i + kBX (j + kBY k), one 64-element
 //                 run per (brick, q): every wavefront load is one contiguous
 //                 64 x sizeof(T) segment (256 B at FP32).
@@ -25,6 +29,7 @@
 namespace lbm355 {
 
 constexpr int kLanes = 64;           // cells per brick = wavefront width
+constexpr int kSpareBricks = 2;      // dummy + sink after the stored bricks
 // brick shape.  One cell in z: the odd AA step's c_z shifts then move whole
 // population runs between bricks instead of splitting their lines, and a
 // wavefront's shifted accesses touch fewer partial lines than with 4x4x4
@@ -68,11 +73,10 @@ struct CaseDev {
     void* strain;             // [active brick][6][64] strain (compute type), or null
     void* force;              // [active brick][3][64] body force (fd-force), or null
     const int32_t* nbr;       // [active brick][27] element offset (brick * 27 * 64) of the neighbour bricks
-    const uint8_t* label;     // [stored brick + dummy][64]
+    const uint8_t* label;     // [stored brick + dummy + sink][64]
     const int32_t* bc_base;   // [active brick] first BC slot of the brick or -1
     const int32_t* bc_cell;   // [nbc] active brick * 64 + lane of each BC slot
     const int32_t* bc_read;   // [nbc][2][27] element offsets of the partner reads
-    const uint32_t* bc_wmask; // [nbc] 27-bit mask of the odd-step writes inside the grid
     const double* bc_factor;  // [nbc] parabolic profile factor, or the inlet area (plug)
     const uint8_t* bc_label;  // [nbc]
     const int16_t* origin;    // [active brick][3] brick coordinates (cells / 4)
@@ -81,11 +85,17 @@ struct CaseDev {
     int64_t stats_bytes;      // size of the statistics buffer (< 4 GiB)
     int64_t stats_epoch;      // step index of the last statistics reset
     int32_t n_active;         // active bricks
-    int32_t n_stored;         // stored bricks (dummy brick index = n_stored)
+    int32_t n_stored;         // stored bricks (dummy brick = n_stored, sink = n_stored + 1)
     int32_t n_bc;
     int32_t nx, ny, nz;
     double nu_ref;            // lattice viscosity of the collision
     double cy_nu0, cy_lambda, cy_a, cy_n;
+    // wave-uniform constants of the neq Carreau-Yasuda solve, computed once at
+    // case creation: the exponent (n - 1) / a, log2 lambda and log2 of the
+    // larger / smaller of the relaxation times 3 nu_ref + 1/2 and 3 nu0 + 1/2
+    double cy_e, cy_log2_lambda, cy_log2_tau_max, cy_log2_tau_min;
+    // the same constants rounded once to FP32 for the FP32 kernels
+    float nu_ref_f, cy_nu0_f, cy_a_f, cy_n_f, cy_e_f, cy_log2_lambda_f, cy_log2_tau_max_f, cy_log2_tau_min_f;
     double inflow_xm, inflow_xp;   // IVC / SVC scale: flux (plug) or u_max (parabolic)
     double pulse_amp, pulse_period;
     int32_t cy_mode;          // 0 off, 1 neq, 2 fd-force (with cy_enabled)

@@ -549,6 +549,34 @@ struct Lg<float> {
     __device__ static float exp2_(float x) { return __builtin_amdgcn_exp2f(x); }
 };
 
+// the wave-uniform Carreau-Yasuda constants of a case in the compute type
+// (precomputed on the host, so no conversion, division or logarithm of
+// uniform values runs per wavefront)
+template <class R>
+struct CyK;
+template <>
+struct CyK<double> {
+    __device__ static double nu_ref(const CaseDev& C) { return C.nu_ref; }
+    __device__ static double nu0(const CaseDev& C) { return C.cy_nu0; }
+    __device__ static double a(const CaseDev& C) { return C.cy_a; }
+    __device__ static double n(const CaseDev& C) { return C.cy_n; }
+    __device__ static double e(const CaseDev& C) { return C.cy_e; }
+    __device__ static double log2_lambda(const CaseDev& C) { return C.cy_log2_lambda; }
+    __device__ static double log2_tau_max(const CaseDev& C) { return C.cy_log2_tau_max; }
+    __device__ static double log2_tau_min(const CaseDev& C) { return C.cy_log2_tau_min; }
+};
+template <>
+struct CyK<float> {
+    __device__ static float nu_ref(const CaseDev& C) { return C.nu_ref_f; }
+    __device__ static float nu0(const CaseDev& C) { return C.cy_nu0_f; }
+    __device__ static float a(const CaseDev& C) { return C.cy_a_f; }
+    __device__ static float n(const CaseDev& C) { return C.cy_n_f; }
+    __device__ static float e(const CaseDev& C) { return C.cy_e_f; }
+    __device__ static float log2_lambda(const CaseDev& C) { return C.cy_log2_lambda_f; }
+    __device__ static float log2_tau_max(const CaseDev& C) { return C.cy_log2_tau_max_f; }
+    __device__ static float log2_tau_min(const CaseDev& C) { return C.cy_log2_tau_min_f; }
+};
+
 // local viscosity of the "neq" coupling: gamma from the deviatoric
 // non-equilibrium second moment P, gamma * tau(gamma) = G = 3 sqrt(2)|P|/(2 rho).
 // Solved for t = log2(gamma) by 3 Newton steps on H(t) = t + log2(tau) - log2(G)
@@ -564,12 +592,12 @@ __device__ __forceinline__ R neq_viscosity(const Moments<R>& M, R ux, R uy, R uz
     const R dxx = Pxx - tr, dyy = Pyy - tr, dzz = Pzz - tr;
     const R s2 = dxx * dxx + dyy * dyy + dzz * dzz + R(2) * (Pxy * Pxy + Pxz * Pxz + Pyz * Pyz);
     const R G = R(2.1213203435596424) * Fm<R>::sqrt_(s2) * Fm<R>::rcp(rho);  // 3 sqrt(2) / 2
-    const R nr = R(C.nu_ref), n0 = R(C.cy_nu0), a = R(C.cy_a), n = R(C.cy_n);
-    const R e = (n - R(1)) / a;
+    using K = CyK<R>;
+    const R nr = K::nu_ref(C), n0 = K::nu0(C), a = K::a(C), n = K::n(C);
+    const R e = K::e(C);
     if (!(G > R(1e-30))) return n0;
-    const R lG = Lg<R>::log2_(G), L = Lg<R>::log2_(R(C.cy_lambda));
-    const R tr_ = R(3) * nr + R(0.5), t0 = R(3) * n0 + R(0.5);
-    const R lo = lG - Lg<R>::log2_(fmax(tr_, t0)), hi = lG - Lg<R>::log2_(fmin(tr_, t0));
+    const R lG = Lg<R>::log2_(G), L = K::log2_lambda(C);
+    const R lo = lG - K::log2_tau_max(C), hi = lG - K::log2_tau_min(C);
     R t = R(0.5) * (lo + hi);
 #pragma unroll
     for (int it = 0; it < 3; ++it) {
@@ -655,7 +683,6 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
 
     R f[27];
     R rho = R(1), ux = R(0), uy = R(0), uz = R(0);
-    uint32_t wmask = 0x7FFFFFFu;
     if (fluid) {
         if (parity == 0) {
 #pragma unroll
@@ -685,7 +712,7 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
         ux = (M.jx + R(0.5) * Fx) * ir;
         uy = (M.jy + R(0.5) * Fy) * ir;
         uz = (M.jz + R(0.5) * Fz) * ir;
-        R nu = R(C.nu_ref);
+        R nu = CyK<R>::nu_ref(C);
         if (C.cy_mode == 1) nu = neq_viscosity<R>(M, ux, uy, uz, C);
         const R om = Fm<R>::rcp(R(3) * nu + R(0.5));
         if constexpr (MODEL == M_CUM) collide_cumulant<R>(f, M, ux, uy, uz, om);
@@ -713,7 +740,6 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
             rho = R(1);
         }
         feq_boundary<R>(f, rho, ux, uy, uz);
-        wmask = C.bc_wmask[slot];
     }
     if (fluid || bc) {
         if (parity == 0) {
@@ -727,9 +753,10 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
                 const int nb = pick_brick(t, cx, cy, cz, (di & ~(kBX - 1)) != 0, (dj & ~(kBY - 1)) != 0,
                                           (dk & ~(kBZ - 1)) != 0);
                 const int l2 = brick_lane(di, dj, dk);
-                // BC cells skip the writes that would leave the grid; the
-                // buffer resource drops them anyway, this keeps them off the bus
-                if ((wmask >> q) & 1u) P::st(F, uint32_t(nb + q * kLanes + l2), q, f[q]);
+                // unconditional: a BC-plane cell's writes that leave the grid
+                // land in an outside lane of its brick or in the sink brick,
+                // neither of which is ever read
+                P::st(F, uint32_t(nb + q * kLanes + l2), q, f[q]);
             }
         }
         // macroscopic output (MacroTcpc::outputMacro, sim_tcpc.cu:337-354)
