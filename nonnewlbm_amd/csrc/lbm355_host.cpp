@@ -312,14 +312,45 @@ int build_index(lbm355_case& c)
     const int X = d.nx, Y = d.ny, Z = d.nz;
     const Labels& L = c.lab;
     // planes 0,1 share a brick for any shift o < B - 1; planes n-2, n-1 do
-    // unless n - 1 + o starts a brick
-    auto pick_shift = [](int n, int B) {
-        for (int o = 0; o < 3 && o < B - 1; ++o)
-            if ((n - 1 + o) % B != 0) return o;
-        return 0;
+    // unless n - 1 + o starts a brick.  Of the shifts that keep both pairs
+    // together, take the one with the fewest active bricks (one wavefront
+    // each): how the 8x8 tiles fall on the vessel cross-sections changes the
+    // count by a few per cent (256x192x128 junction: 7,190 at (0,0), 7,066 at
+    // the best shift).  Counted on runs of non-wall cells along x.
+    auto valid_shift = [](int n, int B, int o) { return o < B - 1 && (n - 1 + o) % B != 0; };
+    struct Run {
+        int y, z, x0, x1;
     };
-    c.ox = pick_shift(X, kBX);
-    c.oy = pick_shift(Y, kBY);
+    std::vector<Run> runs;
+    for (int z = 0; z < Z; ++z)
+        for (int y = 0; y < Y; ++y)
+            for (int x = 0; x < X;) {
+                if (L.map[L.at(x, y, z)] == L_WALL) { ++x; continue; }
+                const int x0 = x;
+                while (x < X && L.map[L.at(x, y, z)] != L_WALL) ++x;
+                runs.push_back(Run{y, z, x0, x});
+            }
+    const int mbx = (X + kBX - 1) / kBX + 1, mby = (Y + kBY - 1) / kBY + 1;
+    const int mbz = (Z + kBZ - 1) / kBZ;
+    std::vector<int32_t> stamp(size_t(mbx) * mby * mbz, -1);
+    int64_t best = -1;
+    c.ox = 0; c.oy = 0;
+    for (int ox = 0; ox < kBX - 1; ++ox) {
+        if (!valid_shift(X, kBX, ox)) continue;
+        for (int oy = 0; oy < kBY - 1; ++oy) {
+            if (!valid_shift(Y, kBY, oy)) continue;
+            const int32_t tag = ox * kBY + oy;
+            int64_t n = 0;
+            for (const Run& r : runs) {
+                const int64_t row = (int64_t(r.z / kBZ) * mby + (r.y + oy) / kBY) * mbx;
+                for (int bx = (r.x0 + ox) / kBX; bx <= (r.x1 - 1 + ox) / kBX; ++bx) {
+                    int32_t& t = stamp[size_t(row + bx)];
+                    if (t != tag) { t = tag; ++n; }
+                }
+            }
+            if (best < 0 || n < best) { best = n; c.ox = ox; c.oy = oy; }
+        }
+    }
     c.nbx = (X + c.ox + kBX - 1) / kBX; c.nby = (Y + c.oy + kBY - 1) / kBY; c.nbz = (Z + kBZ - 1) / kBZ;
     const int64_t NB = int64_t(c.nbx) * c.nby * c.nbz;
     // bricks touched by the AA update: the 3x3x3 neighbourhood of every
