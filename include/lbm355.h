@@ -295,7 +295,8 @@ int lbm355_batch_set_group_bytes(lbm355_batch* b, int64_t bytes);
    *best = bricks_per_group | nontemporal << 8 | waves_per_cu << 16 |
            (group bytes >> 24) << 24 */
 int lbm355_batch_autotune(lbm355_batch* b, int* best);
-/* the same for one case (240 real steps; no grouping) */
+/* the same for one case, with a second round: the 3 fastest configurations are
+   timed again on 2 + 32 steps (342 real steps in total; no grouping) */
 int lbm355_case_autotune(lbm355_case* c, int* best);
 
 /* --- serving pool ----------------------------------------------------------

@@ -579,9 +579,9 @@ struct CyK<float> {
 
 // local viscosity of the "neq" coupling: gamma from the deviatoric
 // non-equilibrium second moment P, gamma * tau(gamma) = G = 3 sqrt(2)|P|/(2 rho).
-// Solved for t = log2(gamma) by 3 Newton steps on H(t) = t + log2(tau) - log2(G)
+// Solved for t = log2(gamma) by Newton steps on H(t) = t + log2(tau) - log2(G)
 // from the middle of the bracket (gamma*tau is close to a power law, so H is
-// nearly linear in t); the oracle runs the same iteration.
+// nearly linear in t); the oracle runs the same iteration (3 steps).
 template <class R>
 __device__ __forceinline__ R neq_viscosity(const Moments<R>& M, R ux, R uy, R uz, const CaseDev& C)
 {
@@ -599,8 +599,12 @@ __device__ __forceinline__ R neq_viscosity(const Moments<R>& M, R ux, R uy, R uz
     const R lG = Lg<R>::log2_(G), L = K::log2_lambda(C);
     const R lo = lG - K::log2_tau_max(C), hi = lG - K::log2_tau_min(C);
     R t = R(0.5) * (lo + hi);
+    // 3 Newton steps reach 1e-14 relative in nu (FP64); 2 reach 8e-7 for the
+    // Carreau-Yasuda sets sx, gx, cy-a and cy-b over gamma = 1e-9..1 (lattice),
+    // the accuracy of the FP32 evaluation itself, so FP32 kernels take 2
+    constexpr int kNewton = sizeof(R) == 4 ? 2 : 3;
 #pragma unroll
-    for (int it = 0; it < 3; ++it) {
+    for (int it = 0; it < kNewton; ++it) {
         const R x = Lg<R>::exp2_(a * (L + t));
         const R b = R(1) + x;
         const R pw = Lg<R>::exp2_(e * Lg<R>::log2_(b));

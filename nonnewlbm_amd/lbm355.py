@@ -91,7 +91,7 @@ class Case:
         check(self.lib.lbm355_case_set_occupancy(self.h, int(waves_per_cu)), "case_set_occupancy")
 
     def autotune(self) -> tuple[int, int, int]:
-        """Time launch shapes x cache policy x wavefront cap on 240 real
+        """Time launch shapes x cache policy x wavefront cap on 342 real
         steps; keeps and returns the fastest (bricks_per_group, nontemporal,
         waves_per_cu)."""
         best = C.c_int()
