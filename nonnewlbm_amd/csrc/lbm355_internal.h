@@ -6,7 +6,7 @@
 //   cells = one wavefront.
 //   Bricks are stored only where the AA update touches memory (the non-wall
 //   cells and their 26-neighbourhood); active bricks (holding at least one
-//   non-wall cell) come first, in Morton order of the brick coordinates, then
+//   non-wall cell) come first, z slab by z slab and row-major in a slab, then
 //   the storage-only shell bricks, then two bricks that stand in for every
 //   brick that is not stored: the "dummy" (index n_stored), read by boundary
 //   partner lookups of absent cells and never written, and the "sink"

@@ -871,7 +871,7 @@ __device__ __forceinline__ void prepass_brick(const StepLaunch& L, const BrickRe
 }
 
 // XCD-aware mapping: workgroups go round-robin to the 8 XCDs; remap them so
-// that each XCD walks one contiguous (Morton-ordered, hence spatially compact)
+// that each XCD walks one contiguous (slab-ordered, hence spatially compact)
 // chunk of the brick list and its L2 sees the neighbour bricks.
 __device__ __forceinline__ int xcd_block(int bid, int grid)
 {
