@@ -49,7 +49,7 @@ __host__ __device__ constexpr int brick_lane(int i, int j, int k)
 constexpr int kQ = 27;               // D3Q27
 constexpr int kStatFields = 6;       // sum rho u (3), sum (rho u)^2 (3)
 constexpr int kMacroFields = 4;      // rho, ux, uy, uz
-constexpr int kStatBlock = 8;        // samples per addition to the FP64 sums (non-FP64 storage)
+constexpr int kStatBlock = 16;       // samples per addition to the FP64 sums (non-FP64 storage)
 constexpr int kRingFields = 3;       // rho u of one sample in a statistics ring slot
 
 // cell labels (the reference map values, sim_tcpc.cu:194-203, plus 7/8 for

@@ -119,8 +119,9 @@ def test_brick_origin_shift(gpu_lib, dims):
 def test_statistics_sums(gpu_lib, storage, every):
     """Mean/rms sums (MacroTcpc e_vm_*, sim_tcpc.cu:347-353) against the
     oracle: read while a group of samples is still open (37 steps), after a
-    call that closes it (40), and after a statistics reset, where the sums of
-    steps 40..59 are the oracle's sums at 60 minus those at 40."""
+    call that closes it (48: a multiple of the 16-sample group, also at
+    stats_every 3), and after a statistics reset, where the sums of steps
+    48..67 are the oracle's sums at 68 minus those at 48."""
     g, kw = channel_case(storage, stats_every=every)
     gpu = Case(g.flat(), g.dims, **kw)
     cpu = OracleCase(g.flat(), g.dims, **dict(kw, storage=_abi.F64))
@@ -140,16 +141,16 @@ def test_statistics_sums(gpu_lib, storage, every):
     mean = gpu.flux(mean=True)
     np.testing.assert_allclose(mean[0], gpu.field("sum_rux")[lab == 3].sum() / n, rtol=1e-9)
     np.testing.assert_allclose(mean[1], gpu.field("sum_rux")[lab == 7].sum() / n, rtol=1e-9)
-    gpu.step(3)
-    cpu.step(3)
-    at40 = {name: cpu.field(name) for name in names}
+    gpu.step(11)
+    cpu.step(11)
+    at48 = {name: cpu.field(name) for name in names}
     for name in names:
-        assert rel_l2(gpu.field(name), at40[name], act) < tol, name
+        assert rel_l2(gpu.field(name), at48[name], act) < tol, name
     gpu.reset_stats()
     gpu.step(20)
     cpu.step(20)
     for name in names:
-        assert rel_l2(gpu.field(name), cpu.field(name) - at40[name], act) < tol, name
+        assert rel_l2(gpu.field(name), cpu.field(name) - at48[name], act) < tol, name
 
 
 @pytest.mark.parametrize("storage", [_abi.F64, _abi.F32])
