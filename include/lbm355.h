@@ -289,14 +289,17 @@ int lbm355_batch_set_occupancy(lbm355_batch* b, int waves_per_cu);
 int lbm355_batch_set_group_bytes(lbm355_batch* b, int64_t bytes);
 /* times every launch shape (1, 2, 4, 8 bricks per workgroup), with and
    without the non-temporal even step, and with no cap, 16 or 12 wavefronts per
-   CU, on 10 real steps each (2 warm-up + 8 timed), then one group, 256 MiB
-   and 128 MiB groups on 18 steps each; all of them count as simulation steps
-   of every member (294 in total).  Keeps the fastest;
+   CU, on real steps: 64 warm-up steps, then per configuration 2 warm-up steps
+   and a timed window W of 16 x stats_every steps (a whole number of statistics
+   groups; 8 without statistics, at most 128), then one group, 256 MiB and
+   128 MiB groups on 2 + 2W steps each.  All of them count as simulation steps
+   of every member (64 + 24 (2 + W) + 3 (2 + 2W)).  Keeps the fastest;
    *best = bricks_per_group | nontemporal << 8 | waves_per_cu << 16 |
            (group bytes >> 24) << 24 */
 int lbm355_batch_autotune(lbm355_batch* b, int* best);
-/* the same for one case, with a second round: the 3 fastest configurations are
-   timed again on 2 + 32 steps (342 real steps in total; no grouping) */
+/* the same for one case without grouping; instead the 3 fastest
+   configurations are timed again, in two interleaved passes of 2 + 2W steps,
+   each keeping its faster pass (64 + 24 (2 + W) + 12 (1 + W) steps) */
 int lbm355_case_autotune(lbm355_case* c, int* best);
 
 /* --- serving pool ----------------------------------------------------------

@@ -1679,6 +1679,16 @@ struct TuneConfig {
 };
 constexpr int kTuneConfigs = 4 * 2 * 3;
 constexpr int kTuneFinalists = 3;   // configurations timed again on longer runs (single cases)
+constexpr int kTuneWarmup = 64;     // steps run before any timing (clocks settle)
+// timed window of the autotuners: a whole number of statistics groups, so
+// every window holds the same number of group flushes (a flush step costs
+// about twice a plain one; 8-step windows of a 16-sample group held one flush
+// or none); 8 steps without statistics, at most 128
+int tune_window(int stats_every)
+{
+    if (stats_every <= 0) return 8;
+    return int(std::min<int64_t>(int64_t(kStatBlock) * stats_every, 128));
+}
 TuneConfig tune_config(int k)
 {
     const int shapes[4] = {1, 2, 4, 8}, caps[3] = {0, 16, 12};
@@ -1915,7 +1925,10 @@ int lbm355_batch_autotune(lbm355_batch* b, int* best)
     // real steps of the batch, an even number per configuration so both AA
     // parities count; the fastest launch shape, cache policy and wavefront
     // cap are kept
-    const int steps = 8;
+    int steps = 8;
+    for (const lbm355_case* c : b->cases) steps = std::max(steps, tune_window(c->d.stats_every));
+    int rc0 = batch_enqueue(b, kTuneWarmup);
+    if (rc0 || (rc0 = batch_finish(b))) return rc0;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1978,13 +1991,15 @@ int lbm355_batch_autotune(lbm355_batch* b, int* best)
 int lbm355_case_autotune(lbm355_case* c, int* best)
 {
     if (!c) return fail(LBM355_EINVAL, "null case");
-    // round 1: every configuration on 2 warm-up + 8 timed steps; round 2: the
-    // kTuneFinalists fastest again on 2 + 32 steps (8-step samples of a ~20 us
-    // step are within launch jitter of each other).  Timed windows are
-    // multiples of the 8-sample statistics group, so each holds the same
-    // number of group flushes.
+    // kTuneWarmup steps first; round 1: every configuration on 2 warm-up steps
+    // and a timed window W (tune_window); round 2: the kTuneFinalists fastest,
+    // two interleaved passes of 2 + 2W steps, each keeping its faster pass
+    // (single windows of a ~20 us step are within launch jitter and clock
+    // drift of each other).
+    const int W = tune_window(c->d.stats_every);
     std::vector<std::pair<double, int>> round1;
     int rc;
+    if ((rc = lbm355_case_step(c, kTuneWarmup))) return rc;
     for (int k = 0; k < kTuneConfigs; ++k) {
         const TuneConfig t = tune_config(k);
         c->bricks_per_group = t.bpg;
@@ -1992,22 +2007,28 @@ int lbm355_case_autotune(lbm355_case* c, int* best)
         c->waves_cap = t.cap;
         if ((rc = lbm355_case_step(c, 2))) return rc;  // warm-up
         double step_ms = 0.0;
-        if ((rc = lbm355_case_time_kernel(c, 8, &step_ms, nullptr))) return rc;
+        if ((rc = lbm355_case_time_kernel(c, W, &step_ms, nullptr))) return rc;
         round1.push_back({step_ms, k});
     }
     std::sort(round1.begin(), round1.end());
-    double best_ms = 0.0;
-    int best_bpg = c->bricks_per_group, best_nt = c->nontemporal, best_cap = c->waves_cap;
-    for (int r = 0; r < kTuneFinalists && r < int(round1.size()); ++r) {
-        const TuneConfig t = tune_config(round1[size_t(r)].second);
-        c->bricks_per_group = t.bpg;
-        c->nontemporal = t.nt;
-        c->waves_cap = t.cap;
-        if ((rc = lbm355_case_step(c, 2))) return rc;
-        double step_ms = 0.0;
-        if ((rc = lbm355_case_time_kernel(c, 32, &step_ms, nullptr))) return rc;
-        if (r == 0 || step_ms < best_ms) { best_ms = step_ms; best_bpg = t.bpg; best_nt = t.nt; best_cap = t.cap; }
-    }
+    const int nf = std::min(kTuneFinalists, int(round1.size()));
+    std::vector<double> fin(size_t(nf), 0.0);
+    for (int pass = 0; pass < 2; ++pass)
+        for (int r = 0; r < nf; ++r) {
+            const TuneConfig t = tune_config(round1[size_t(r)].second);
+            c->bricks_per_group = t.bpg;
+            c->nontemporal = t.nt;
+            c->waves_cap = t.cap;
+            if ((rc = lbm355_case_step(c, 2))) return rc;
+            double step_ms = 0.0;
+            if ((rc = lbm355_case_time_kernel(c, 2 * W, &step_ms, nullptr))) return rc;
+            if (pass == 0 || step_ms < fin[size_t(r)]) fin[size_t(r)] = step_ms;
+        }
+    int best_r = 0;
+    for (int r = 1; r < nf; ++r)
+        if (fin[size_t(r)] < fin[size_t(best_r)]) best_r = r;
+    const TuneConfig bt = tune_config(round1[size_t(best_r)].second);
+    const int best_bpg = bt.bpg, best_nt = bt.nt, best_cap = bt.cap;
     c->bricks_per_group = best_bpg;
     c->nontemporal = best_nt;
     c->waves_cap = best_cap;

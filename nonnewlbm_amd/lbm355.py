@@ -91,9 +91,9 @@ class Case:
         check(self.lib.lbm355_case_set_occupancy(self.h, int(waves_per_cu)), "case_set_occupancy")
 
     def autotune(self) -> tuple[int, int, int]:
-        """Time launch shapes x cache policy x wavefront cap on 342 real
-        steps; keeps and returns the fastest (bricks_per_group, nontemporal,
-        waves_per_cu)."""
+        """Time launch shapes x cache policy x wavefront cap on real steps
+        (700 with statistics every step); keeps and returns the fastest
+        (bricks_per_group, nontemporal, waves_per_cu)."""
         best = C.c_int()
         check(self.lib.lbm355_case_autotune(self.h, C.byref(best)), "case_autotune")
         return best.value & 0xFF, (best.value >> 8) & 0xFF, best.value >> 16
@@ -193,7 +193,7 @@ class Batch:
 
     def autotune(self) -> tuple[int, int, int, int]:
         """Returns the fastest (bricks_per_group, nontemporal, waves_per_cu,
-        group MiB with 0 = one group), now in use; 294 real steps."""
+        group MiB with 0 = one group), now in use; 598 real steps with statistics every step."""
         best = C.c_int()
         check(self.lib.lbm355_batch_autotune(self.h, C.byref(best)), "batch_autotune")
         v = best.value
