@@ -645,6 +645,17 @@ int64_t case_device_bytes(const lbm355_case& c)
     return bytes;
 }
 
+// device memory a step of the case sweeps: all of it except the statistics
+// ring, of which a step touches one slot (the group's flush reads the rest
+// once every kStatBlock samples).  Cache-residency decisions (the automatic
+// non-temporal even step, batch groups, the pool's concurrent ticks) use this;
+// admission against a memory budget uses case_device_bytes.
+int64_t case_step_bytes(const lbm355_case& c)
+{
+    const int64_t ring = int64_t(c.stats_part.bytes);
+    return case_device_bytes(c) - ring + ring / (kStatBlock - 1);
+}
+
 // one time step of a case: the fd-force pre-kernels (velocity, strain, body
 // force) when that Carreau-Yasuda mode is on, then the fused step
 int launch_case_step(lbm355_case& c, const StepLaunch& L)
@@ -693,7 +704,7 @@ StepLaunch single_launch(lbm355_case& c)
     L.base = static_cast<const int64_t*>(c.base.p);
     L.rel = c.steps;
     L.bricks_per_group = c.bricks_per_group;
-    L.nontemporal = c.nontemporal >= 0 ? c.nontemporal : (case_device_bytes(c) > kNontemporalBytes ? 1 : 0);
+    L.nontemporal = c.nontemporal >= 0 ? c.nontemporal : (case_step_bytes(c) > kNontemporalBytes ? 1 : 0);
     L.lds_bytes = occupancy_lds(c.waves_cap, c.bricks_per_group);
     return L;
 }
@@ -1654,7 +1665,7 @@ struct lbm355_batch {
     };
     std::vector<Group> groups;
     std::vector<int> case_first;        // brick list offset of each member
-    std::vector<int64_t> case_bytes;    // device memory of each member
+    std::vector<int64_t> case_bytes;    // device memory a step of each member sweeps (case_step_bytes)
     int64_t group_bytes = kBatchGroupBytes;
     int64_t launches = 0;     // steps done by the batch
     int64_t base_host = 0;    // mirror of the device launch base
@@ -1763,7 +1774,7 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
         b->start.push_back(-c->steps);
         b->epoch.push_back(c->stats_first);
         b->case_first.push_back(int(list.size()));
-        b->case_bytes.push_back(case_device_bytes(*c));
+        b->case_bytes.push_back(case_step_bytes(*c));
         for (int k = 0; k < c->n_active; ++k) list.push_back(BrickRef{uint32_t(i), uint32_t(k)});
     }
     // the cases' slab-ordered brick runs follow each other; the
@@ -1816,6 +1827,8 @@ int lbm355_batch_set_occupancy(lbm355_batch* b, int waves_per_cu)
 
 // the library's error channel for the other host sources
 int lbm355_set_error(int code, const std::string& msg) { return fail(code, msg); }
+// device memory a step of the case sweeps, for the pool's scheduling
+int64_t lbm355_case_step_bytes(const lbm355_case* c) { return c ? case_step_bytes(*c) : 0; }
 
 // queue n steps of a batch on its stream without waiting (the pool enqueues
 // every batch of a tick, then finishes them)

@@ -32,6 +32,7 @@
 int batch_enqueue(lbm355_batch* b, int64_t n);
 int batch_finish(lbm355_batch* b);
 int lbm355_set_error(int code, const std::string& msg);
+int64_t lbm355_case_step_bytes(const lbm355_case* c);
 
 namespace {
 
@@ -206,7 +207,7 @@ void lbm355_pool::run()
         // they run one after another, so that each batch's cache-sized groups
         // do not evict each other
         int64_t resident = 0;
-        for (Request* r : running) resident += r->st.device_bytes;
+        for (Request* r : running) resident += lbm355_case_step_bytes(r->c);
         const bool serial = resident > lbm355::kBatchGroupBytes;
         for (lbm355_batch* b : batches) {
             if (rc == LBM355_OK) rc = batch_enqueue(b, n);
