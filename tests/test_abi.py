@@ -114,3 +114,32 @@ def test_case_above_4gib_is_refused_before_the_device():
     assert rc == _abi.EINVAL and not h.value
     assert "4 GiB" in lib.lbm355_last_error().decode()
     del keep
+
+
+def _min_active_bricks(lab: np.ndarray) -> int:
+    """Fewest 8x8x1 bricks holding a non-wall cell over every brick-origin
+    shift (ox, oy) that keeps the planes 0,1 and n-2,n-1 in one brick:
+    0 <= o <= 6 and n - 1 + o not a multiple of 8 (counted by brute force
+    on the label field, independently of the library's run-based count)."""
+    X, Y, Z = lab.shape
+    x, y, z = np.nonzero(lab != 1)
+    best = None
+    for ox in range(7):
+        if (X - 1 + ox) % 8 == 0:
+            continue
+        for oy in range(7):
+            if (Y - 1 + oy) % 8 == 0:
+                continue
+            keys = np.unique((z.astype(np.int64) * 4096 + (y + oy) // 8) * 4096 + (x + ox) // 8)
+            best = len(keys) if best is None else min(best, len(keys))
+    return best
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+def test_origin_shift_minimises_active_bricks(name, make):
+    """Case setup picks, among the brick-origin shifts that keep each BC plane
+    pair in one brick, one with the fewest active bricks (one wavefront each)."""
+    g, kw = make()
+    lab, info = lbm355.labels(g.flat(), g.dims, **kw)
+    assert lab.shape == tuple(g.dims)
+    assert info["bricks_active"] == _min_active_bricks(lab)
