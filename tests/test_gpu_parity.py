@@ -153,6 +153,30 @@ def test_statistics_sums(gpu_lib, storage, every):
         assert rel_l2(gpu.field(name), cpu.field(name) - at48[name], act) < tol, name
 
 
+@pytest.mark.parametrize("storage", [_abi.F16_OFFSET, _abi.BF16_OFFSET])
+def test_statistics_ring_half_storage(gpu_lib, storage):
+    """With 2-byte population storage the statistics go through the same FP32
+    sample ring: the sums equal FP64 sums of the FP32 samples rho*u that the
+    step's own macro output yields, stepped one step at a time, read with a
+    group open (40 samples) and closed (48)."""
+    g, kw = channel_case(storage, stats_every=1)
+    gpu = Case(g.flat(), g.dims, **kw)
+    act = gpu.field("label") != 1
+    acc = {a: np.zeros(act.sum()) for a in ("x", "y", "z")}
+    sq = {a: np.zeros(act.sum()) for a in ("x", "y", "z")}
+    for step in range(1, 49):
+        gpu.step(1)
+        rho = gpu.field("rho")[act].astype(np.float32)
+        for a in ("x", "y", "z"):
+            v = (rho * gpu.field("v" + a)[act].astype(np.float32)).astype(np.float64)
+            acc[a] += v
+            sq[a] += v * v
+        if step in (40, 48):
+            for a in ("x", "y", "z"):
+                np.testing.assert_allclose(gpu.field("sum_ru" + a)[act], acc[a], rtol=1e-12, atol=1e-15)
+                np.testing.assert_allclose(gpu.field("sum_rru" + a * 2)[act], sq[a], rtol=1e-12, atol=1e-15)
+
+
 @pytest.mark.parametrize("storage", [_abi.F64, _abi.F32])
 @pytest.mark.parametrize("collision", [_abi.CUM, _abi.KBC])
 def test_carreau_yasuda_neq(gpu_lib, storage, collision):
