@@ -55,8 +55,13 @@ enum {
 enum {
     LBM355_F64 = 0,         /* TraitsDP (solver/lbm3d/defs.h:141)                */
     LBM355_F32 = 1,         /* TraitsSP, the sim_tcpc default (defs.h:140)       */
-    LBM355_F16_OFFSET = 2,  /* f_q - w_q stored as IEEE half, FP32 arithmetic    */
-    LBM355_BF16_OFFSET = 3  /* f_q - w_q stored as bfloat16, FP32 arithmetic     */
+    LBM355_F16_OFFSET = 2,  /* (f_q - w_q) 2^e_q stored as IEEE half (w_q 2^e_q =
+                               16/27), stochastically rounded, FP32 arithmetic  */
+    LBM355_BF16_OFFSET = 3  /* f_q - w_q stored as bfloat16, stochastically
+                               rounded, FP32 arithmetic; for precision studies
+                               only: it misses the 1e-3 velocity bar (4.3e-3 on
+                               the 512x384x256 junction after 40,000 steps), so
+                               the pool and the sim_tcpc CLI refuse it          */
 };
 
 /* Carreau-Yasuda coupling */
@@ -343,7 +348,8 @@ typedef struct lbm355_pool_info {
    graph = 1 replays captured hipGraphs inside a batch */
 int lbm355_pool_create(int64_t budget_bytes, int steps_per_tick, int graph, lbm355_pool** out);
 /* copies the descriptor and geometry; report_every > 0 reports plane fluxes
-   every so many steps; keep = 1 keeps the finished case for lbm355_pool_take */
+   every so many steps; keep = 1 keeps the finished case for lbm355_pool_take.
+   LBM355_EINVAL for LBM355_BF16_OFFSET storage (not admitted for serving) */
 int lbm355_pool_submit(lbm355_pool* p, const lbm355_case_desc* desc, int64_t steps, int64_t report_every,
                        int keep, int64_t* id);
 /* LBM355_OK when the request is done or failed, LBM355_EBUSY while queued or running */

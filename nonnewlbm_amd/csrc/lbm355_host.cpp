@@ -197,6 +197,9 @@ struct lbm355_case {
     bool strain_valid = false;
 
     hipStream_t stream = nullptr;
+    // recorded on the case's stream when a batch takes the case over, so that
+    // the batch's steps start after everything already queued for the case
+    hipEvent_t joined = nullptr;
     DevBuf f, stats, stats_part, macro, strain, force, nbr, label, bc_base, bc_cell_d, bc_read, bc_factor, bc_label,
         origin, base, devcase, list, flux;
     CaseDev host{};
@@ -207,6 +210,7 @@ struct lbm355_case {
     ~lbm355_case()
     {
         if (stream) { (void)hipStreamSynchronize(stream); (void)hipStreamDestroy(stream); }
+        if (joined) (void)hipEventDestroy(joined);
     }
 
     int64_t cell_index(int x, int y, int z) const { return x + int64_t(d.nx) * (y + int64_t(d.ny) * z); }
@@ -427,6 +431,7 @@ int setup_device(lbm355_case& c)
     const int X = d.nx, Y = d.ny, Z = d.nz;
     const int na = c.n_active, ns = c.n_stored;
     HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c.joined, hipEventDisableTiming));
 
     // labels per stored brick (+ dummy and sink = wall)
     std::vector<uint8_t> lab(size_t(ns + kSpareBricks) * kLanes, L_WALL);
@@ -1133,7 +1138,7 @@ int lbm355_case_get_populations(lbm355_case* c, double* host, size_t bytes)
             } else if (e == 31) bits = sgn | 0x7F800000u | m << 13;
             else bits = sgn | uint32_t(e + 112) << 23 | m << 13;
             float v; std::memcpy(&v, &bits, 4);
-            return double(v + float(lattice_weight(q)));
+            return double(std::fma(v, 1.0f / float(1 << half_scale_exp(q)), float(lattice_weight(q))));
         }
         default: {
             uint16_t h; std::memcpy(&h, &raw[size_t(off) * 2], 2);
@@ -1393,7 +1398,7 @@ int set_populations(lbm355_case* c, const std::vector<double>& pops)
         case LBM355_F64: std::memcpy(&raw[size_t(off) * 8], &v, 8); break;
         case LBM355_F32: { const float f = float(v); std::memcpy(&raw[size_t(off) * 4], &f, 4); break; }
         case LBM355_F16_OFFSET: {
-            const uint16_t h = host_f16(float(v) - float(lattice_weight(q)));
+            const uint16_t h = host_f16((float(v) - float(lattice_weight(q))) * float(1 << half_scale_exp(q)));
             std::memcpy(&raw[size_t(off) * 2], &h, 2);
             break;
         }
@@ -1671,6 +1676,10 @@ struct lbm355_batch {
     int64_t base_host = 0;    // mirror of the device launch base
     std::vector<int64_t> start;  // per case: launch index of its step 0
     std::vector<int64_t> epoch;  // per case: statistics epoch in the batch copy
+    std::vector<double> in_xm, in_xp;  // per case: inflow scales in the batch copy
+    // recorded on the batch's stream after its steps; every member's stream
+    // waits on it, so whatever is queued for a case later runs after them
+    hipEvent_t done = nullptr;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_steps = 0;
@@ -1679,6 +1688,7 @@ struct lbm355_batch {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (stream) { (void)hipStreamSynchronize(stream); (void)hipStreamDestroy(stream); }
+        if (done) (void)hipEventDestroy(done);
     }
 };
 
@@ -1773,6 +1783,8 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
         cd.back().start = -c->steps;  // batch launch 0 is the case's step c->steps
         b->start.push_back(-c->steps);
         b->epoch.push_back(c->stats_first);
+        b->in_xm.push_back(c->host.inflow_xm);
+        b->in_xp.push_back(c->host.inflow_xp);
         b->case_first.push_back(int(list.size()));
         b->case_bytes.push_back(case_step_bytes(*c));
         for (int k = 0; k < c->n_active; ++k) list.push_back(BrickRef{uint32_t(i), uint32_t(k)});
@@ -1782,6 +1794,7 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
     b->n_list = int(list.size());
     regroup(*b);
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
     int rc;
     if ((rc = upload(b->devcases, cd))) return rc;
     if ((rc = upload(b->list, list))) return rc;
@@ -1836,6 +1849,13 @@ int batch_enqueue(lbm355_batch* b, int64_t n)
 {
     if (!b) return fail(LBM355_EINVAL, "null batch");
     if (n <= 0) return LBM355_OK;
+    // the batch's steps follow whatever is queued on each member's own stream
+    // (steps, statistics resets, inflow updates): the streams are
+    // independent, so without this wait both could touch one case at once
+    for (lbm355_case* c : b->cases) {
+        HIP_TRY(hipEventRecord(c->joined, c->stream));
+        HIP_TRY(hipStreamWaitEvent(b->stream, c->joined, 0));
+    }
     for (size_t i = 0; i < b->cases.size(); ++i) {
         CaseDev* dc = static_cast<CaseDev*>(b->devcases.p) + i;
         // a case stepped on its own since the last batch call: re-anchor it
@@ -1847,6 +1867,21 @@ int batch_enqueue(lbm355_batch* b, int64_t n)
         if (b->epoch[i] != b->cases[i]->stats_first) {
             LAUNCH_TRY(launch_set_u64(&dc->stats_epoch, uint64_t(b->cases[i]->stats_first), b->stream));
             b->epoch[i] = b->cases[i]->stats_first;
+        }
+        // the per-step inflow hook (lbm355_case_set_inflow, updateKernelVelocities
+        // of sim_tcpc.cu:657-663) writes the case's own copy of its constants
+        const CaseDev& h = b->cases[i]->host;
+        if (h.inflow_xm != b->in_xm[i]) {
+            uint64_t v;
+            std::memcpy(&v, &h.inflow_xm, 8);
+            LAUNCH_TRY(launch_set_u64(&dc->inflow_xm, v, b->stream));
+            b->in_xm[i] = h.inflow_xm;
+        }
+        if (h.inflow_xp != b->in_xp[i]) {
+            uint64_t v;
+            std::memcpy(&v, &h.inflow_xp, 8);
+            LAUNCH_TRY(launch_set_u64(&dc->inflow_xp, v, b->stream));
+            b->in_xp[i] = h.inflow_xp;
         }
         // rho,u of every case at the last step of this call
         LAUNCH_TRY(launch_set_u64(&dc->macro_step, uint64_t(b->cases[i]->steps + n - 1), b->stream));
@@ -1902,6 +1937,9 @@ int batch_enqueue(lbm355_batch* b, int64_t n)
         }
         b->launches += n - done;
     }
+    // and whatever is queued for a member afterwards follows the batch's steps
+    HIP_TRY(hipEventRecord(b->done, b->stream));
+    for (lbm355_case* c : b->cases) HIP_TRY(hipStreamWaitEvent(c->stream, b->done, 0));
     for (lbm355_case* c : b->cases) {
         if (c->d.stats_every > 0) {
             const int64_t k = c->d.stats_every;

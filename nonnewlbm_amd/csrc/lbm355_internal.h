@@ -146,6 +146,18 @@ __host__ __device__ constexpr double lattice_weight(int q)
                : 1.0 / 216.0;
 }
 
+// FP16-offset storage keeps (f_q - w_q) * 2^e_q in IEEE half, e_q chosen per
+// velocity class so that w_q * 2^e_q = 16/27 for every q: the stored value is
+// (f_q / w_q - 1) * 16/27, the same scale for all 27 directions, and small
+// deviations of the 1/216 corner weights stay out of the half subnormals
+__host__ __device__ constexpr int half_scale_exp(int q)
+{
+    return ((q / 9 - 1) * (q / 9 - 1) + ((q / 3) % 3 - 1) * ((q / 3) % 3 - 1) + (q % 3 - 1) * (q % 3 - 1)) == 0 ? 1
+           : ((q / 9 - 1) * (q / 9 - 1) + ((q / 3) % 3 - 1) * ((q / 3) % 3 - 1) + (q % 3 - 1) * (q % 3 - 1)) == 1 ? 3
+           : ((q / 9 - 1) * (q / 9 - 1) + ((q / 3) % 3 - 1) * ((q / 3) % 3 - 1) + (q % 3 - 1) * (q % 3 - 1)) == 2 ? 5
+                                                                                                              : 7;
+}
+
 // launchers (lbm355_kernels.hip); all asynchronous on `stream`
 // one fused step over the launch's brick list
 int launch_step(int storage, int model, const StepLaunch& L, void* stream);

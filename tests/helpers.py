@@ -28,6 +28,39 @@ def tcpc_case(scale=0.25, storage=_abi.F32, collision=_abi.CUM, **extra):
     return g, kw
 
 
+def stored_populations(act: np.ndarray, steps: int) -> np.ndarray:
+    """[27, n] mask over the populations of the active cells ``act`` (in the
+    order of ``pops[:, act]``) that the AA layout holds after ``steps`` steps.
+
+    After an odd number of steps (the last one even) population q of a cell
+    sits in the cell's own slot opp(q): all are held.  After an even number
+    (the last one odd) it sits in slot q of the downstream cell x + c_q;
+    where that cell is outside the grid the population has no slot.  Those
+    are exactly the outward directions of the cells on the grid faces, which
+    no step reads (the clamped pull and the boundary partner reads only take
+    inward and tangential directions), and the library reports them as 0."""
+    idx = np.nonzero(act)
+    held = np.ones((27, idx[0].size), dtype=bool)
+    if steps % 2 == 1:
+        return held
+    for q in range(27):
+        c = (q // 9 - 1, (q // 3) % 3 - 1, q % 3 - 1)
+        for a in range(3):
+            t = idx[a] + c[a]
+            held[q] &= (t >= 0) & (t < act.shape[a])
+    return held
+
+
+def assert_populations_match(pg: np.ndarray, pc: np.ndarray, act: np.ndarray, steps: int, atol: float) -> None:
+    """GPU populations ``pg`` against the oracle's ``pc`` ([27, X, Y, Z]) on
+    every active cell: the held ones to ``atol``, the unheld ones exactly 0."""
+    held = stored_populations(act, steps)
+    a, b = pg[:, act], pc[:, act]
+    assert np.all(a[~held] == 0.0), "a population without a slot is reported non-zero"
+    worst = float(np.max(np.abs(a[held] - b[held])))
+    assert worst < atol, worst
+
+
 def rel_l2(a: np.ndarray, b: np.ndarray, mask: np.ndarray | None = None) -> float:
     if mask is not None:
         a = a[mask]

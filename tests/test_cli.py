@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import shlex
 import subprocess
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -28,7 +29,7 @@ def test_run_case_argv_closed_form(capsys):
                        "--block-size", "32"]
     assert build_command(ROOT, "v4a", "gx", "x", lbm_viscosity=1e-3)[7:11] == ["--ivc-mls", "33.33333333",
                                                                                 "--svc-mls", "18.33333333"]
-    with pytest.raises(ValueError, match="Unknown regime"):
+    with pytest.raises(ValueError, match="regime .v9. is not in the regime table"):
         build_command(ROOT, "v9", "sx")
     assert run_case_main(["--regime", "v2", "--material", "newtonian", "--dry-run", "--precision", "fp16"]) == 0
     out = capsys.readouterr().out.strip()
@@ -94,7 +95,7 @@ def test_flows_known_velocity(tmp_path):
     assert row["mass_balance_l_min"] == pytest.approx((1.2 + 0.4 - 0.56 - 0.64) * a)
     assert row["lpa_split_percent"] == pytest.approx(100 * 0.64 / 1.2)
     assert row["rpa_split_percent"] + row["lpa_split_percent"] == pytest.approx(100.0)
-    assert flows.parse_label_map("3:IVC,7:OUT") == {3: "IVC", 7: "OUT"}
+    assert flows.parse_labels("3:IVC,7:OUT") == {3: "IVC", 7: "OUT"}
 
 
 def test_run_all_dry_run_and_helpers(capsys):
@@ -131,3 +132,55 @@ def test_run_all_dry_run_and_helpers(capsys):
     row = run_all.flows_row(flux, {"area_ivc": 6, "area_svc": 4, "area_out5": 4, "area_out6": 4}, dl, dt)
     for k, v in row.items():
         assert v == pytest.approx(ref[k], rel=1e-12, abs=1e-15), k
+
+
+def _write_point_vtk(path, wall, vel, dx):
+    """A minimal sim_tcpc-style output file (legacy VTK, binary big-endian,
+    RECTILINEAR_GRID, POINT_DATA wall + velocity)."""
+    X, Y, Z = wall.shape
+    with open(path, "wb") as f:
+        f.write(b"# vtk DataFile Version 3.0\ntest\nBINARY\nDATASET RECTILINEAR_GRID\n")
+        f.write(f"DIMENSIONS {X} {Y} {Z}\n".encode())
+        for axis, n in zip("XYZ", (X, Y, Z)):
+            f.write(f"{axis}_COORDINATES {n} float\n".encode())
+            f.write((np.arange(n) * dx).astype(">f4").tobytes())
+        f.write(f"POINT_DATA {X * Y * Z}\nSCALARS wall int 1\nLOOKUP_TABLE default\n".encode())
+        f.write(wall.transpose(2, 1, 0).astype(">i4").tobytes())
+        f.write(b"VECTORS velocity float\n")
+        f.write(vel.transpose(2, 1, 0, 3).astype(">f4").tobytes())
+
+
+def test_flows_command_line_and_columns(tmp_path):
+    """The compute_flows.py command line (positional files or globs,
+    --labels, --output; prints the CSV path) and its CSV columns in order,
+    one row per snapshot in numeric order."""
+    X, Y, Z = 12, 10, 8
+    wall = np.ones((X, Y, Z), dtype=np.int32)
+    wall[0, 3:5, 3:5] = 3
+    wall[X - 1, 4:6, 3:5] = 4
+    wall[5:7, 0, 3:5] = 5
+    wall[5:7, Y - 1, 3:5] = 6
+    for k in (2, 10):   # rank000_data_10 sorts after _2
+        vel = np.zeros((X, Y, Z, 3))
+        vel[0, 3:5, 3:5, 0] = 0.1 * k
+        vel[5:7, Y - 1, 3:5, 1] = 0.05 * k
+        _write_point_vtk(tmp_path / f"rank000_data_{k}.vtk", wall, vel, 2e-3)
+    out = tmp_path / "f.csv"
+    r = subprocess.run([sys.executable, "-m", "nonnewlbm_amd.flows", str(tmp_path / "rank000_data_*.vtk"),
+                        "--output", str(out)], cwd=ROOT, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == str(out)
+    lines = out.read_text().splitlines()
+    assert lines[0].split(",") == ["file", "IVC_l_min", "IVC_cells", "SVC_l_min", "SVC_cells", "RPA_l_min",
+                                   "RPA_cells", "LPA_l_min", "LPA_cells", "mass_balance_l_min",
+                                   "lpa_split_percent", "rpa_split_percent"]
+    rows = [ln.split(",") for ln in lines[1:]]
+    assert [Path(r[0]).name for r in rows] == ["rank000_data_2.vtk", "rank000_data_10.vtk"]
+    a = 2e-3 * 2e-3 * 6e4
+    assert float(rows[1][1]) == pytest.approx(4 * 1.0 * a, rel=1e-6) and rows[1][2] == "4"
+    assert float(rows[1][10]) == pytest.approx(100.0)
+    r = subprocess.run([sys.executable, "-m", "nonnewlbm_amd.flows", str(tmp_path / "rank000_data_2.vtk"),
+                        "--labels", "3:IVC,6:LPA", "--output", str(out)], cwd=ROOT, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0 and out.read_text().splitlines()[0].split(",")[:5] == [
+        "file", "IVC_l_min", "IVC_cells", "LPA_l_min", "LPA_cells"]

@@ -91,3 +91,55 @@ def test_autotune_keeps_results(gpu_lib):
     ref[0].step(m + 74 - n)
     for name in ("rho", "vx"):
         np.testing.assert_array_equal(one.field(name), ref[0].field(name))
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_follows_inflow_hook(gpu_lib, graph):
+    """updateKernelVelocities (sim_tcpc.cu:657-663) on batched cases: the
+    caval fluxes of one member change between batch calls (after 70 steps,
+    which include a 64-step graph replay, then after 64 more); every member
+    equals the same case stepped alone with the same changes."""
+    specs = _cases(_abi.F32)
+    alone = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
+    mixed = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
+    b = Batch(mixed, graph=graph)
+    for n, scale in ((70, 1.7), (64, 0.4), (9, None)):
+        b.step(n)
+        for c in alone:
+            c.step(n)
+        if scale is not None:
+            kw = specs[1][1]
+            for c in (mixed[1], alone[1]):
+                c.set_inflow(kw["ivc_flux"] * scale, kw["svc_flux"] * (2.0 - scale))
+    b.close()
+    for k in range(3):
+        for name in ("rho", "vx", "vy", "vz"):
+            np.testing.assert_array_equal(mixed[k].field(name), alone[k].field(name), err_msg=f"case {k} {name}")
+    fl = mixed[1].flux(mean=False)
+    assert fl[0] == pytest.approx(specs[1][1]["ivc_flux"] * 0.4, rel=1e-5)
+
+
+def test_batch_waits_for_work_queued_on_a_case(gpu_lib):
+    """Steps and a statistics reset queued on a member's own stream right
+    before a batch call (no explicit synchronisation in between) complete
+    before the batch touches the case, and the case's own later steps follow
+    the batch's: the result equals the case stepped alone."""
+    specs = _cases(_abi.F32)
+    alone = [Case(g.flat(), g.dims, **dict(kw, stats_every=1)) for g, kw in specs]
+    mixed = [Case(g.flat(), g.dims, **dict(kw, stats_every=1)) for g, kw in specs]
+    b = Batch(mixed, graph=True)
+    b.step(3)
+    mixed[0].step(40)          # queued on case 0's stream, not waited for
+    mixed[2].reset_stats()
+    b.step(66)
+    mixed[2].step(5)           # queued behind the batch's steps
+    b.step(2)
+    alone[0].step(3 + 40 + 66 + 2)
+    alone[1].step(3 + 66 + 2)
+    alone[2].step(3)
+    alone[2].reset_stats()
+    alone[2].step(66 + 5 + 2)
+    b.close()
+    for k in range(3):
+        for name in ("rho", "vx", "vy", "sum_rux", "sum_rruyy"):
+            np.testing.assert_array_equal(mixed[k].field(name), alone[k].field(name), err_msg=f"case {k} {name}")

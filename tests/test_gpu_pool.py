@@ -20,7 +20,7 @@ def _specs():
                                                  (0.3, _abi.F16_OFFSET, _abi.KBC, 33, None),
                                                  (0.25, _abi.F32, _abi.CUM, 41, "cy-a"),
                                                  (0.3, _abi.F32, _abi.BGK, 57, None),
-                                                 (0.25, _abi.BF16_OFFSET, _abi.MRT, 20, None),
+                                                 (0.25, _abi.F16_OFFSET, _abi.MRT, 20, None),
                                                  (0.25, _abi.F32, _abi.CUM, 29, "sx-fd")):
         g, kw = tcpc_case(scale, storage=storage, collision=collision)
         if cy == "sx-fd":   # the reference's fd-force mode steps outside the batches
@@ -112,4 +112,14 @@ def test_pool_large_cases_step_batches_in_turn(gpu_lib):
             np.testing.assert_array_equal(got.field(name), alone.field(name), err_msg=f"{rid} {name}")
         got.close()
         alone.close()
+    pool.close()
+
+
+def test_pool_refuses_bfloat16(gpu_lib):
+    """bfloat16-offset storage misses the 1e-3 velocity bar against FP64, so
+    the pool does not serve it (a documented EINVAL at submit)."""
+    g, kw = tcpc_case(0.25, storage=_abi.BF16_OFFSET)
+    pool = Pool(steps_per_tick=8)
+    with pytest.raises(RuntimeError, match="bfloat16"):
+        pool.submit(g.flat(), g.dims, 10, **kw)
     pool.close()

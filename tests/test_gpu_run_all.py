@@ -74,7 +74,7 @@ def test_run_all_processes_and_harvest(gpu_lib, tmp_path):
     (row,) = _summary(tmp_path)
     assert row["case_id"] == "v1_newtonian" and row["state"] == "COMPLETED" and row["harvest_status"] == "ok"
     vtks = sorted(glob.glob(str(tmp_path / "results_v1_newtonian" / "vtk3D" / "rank000_data_*.vtk")),
-                  key=flows.natural_key)
+                  key=flows.snapshot_order)
     assert int(row["snapshot_count"]) == len(vtks) >= 2
     ref = flows.compute_file(Path(vtks[-1]))
     for k in ("IVC_l_min", "SVC_l_min", "RPA_l_min", "LPA_l_min", "mass_balance_l_min", "lpa_split_percent"):

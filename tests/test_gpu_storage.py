@@ -1,0 +1,87 @@
+"""Reduced-precision population storage against the FP64 oracle.
+
+The TCPC junction at scale 0.25 (64x48x32) with the cumulant collision,
+Carreau-Yasuda set A in the local (neq) coupling and pulsatile parabolic
+inflows u(t) = u (1 + 0.3 sin(2 pi t / 400)), 2,000 steps: the CPU oracle runs
+it in FP64, the GPU stores the populations as FP32, FP16-offset or
+BF16-offset (FP32 arithmetic).  Each stored type states its error: relative L2
+over the non-wall cells of velocity, of rho - 1, of the shear rate of the
+output path (FD strain, core.h:407-576) and the LPA share of the mean outlet
+flow (compute_flows.py:141-147 on the label-5/6 planes).
+
+Tolerances (about twice the errors measured on the MI355X): FP32 velocity
+1e-4; FP16-offset velocity 1e-3 and shear rate 2e-3; BF16-offset velocity
+5e-3 (it is refused for serving); the flow split within 0.5 % (in share
+points) for every type.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from nonnewlbm_amd import _abi
+from nonnewlbm_amd.lbm355 import Case
+from oracle.oracle import OracleCase
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 2000
+# storage -> (velocity, rho - 1, shear rate) relative L2 bounds
+TOL = {_abi.F32: (1e-4, 1e-4, 2e-4), _abi.F16_OFFSET: (1e-3, 1e-3, 2e-3), _abi.BF16_OFFSET: (5e-3, 5e-3, 1e-2)}
+
+
+def _case(storage):
+    import bench
+    g, kw = bench.make_case(0.25, storage, _abi.CUM, 0.01)
+    kw.update(pulse_amp=0.3, pulse_period=400.0)
+    return g, kw
+
+
+def _gamma(c):
+    s = {k: c.field(k) for k in ("s11", "s12", "s13", "s22", "s32", "s33")}
+    return np.sqrt(2.0) * np.sqrt(s["s11"] ** 2 + s["s22"] ** 2 + s["s33"] ** 2
+                                  + 2 * (s["s12"] ** 2 + s["s13"] ** 2 + s["s32"] ** 2))
+
+
+@pytest.fixture(scope="module")
+def reference():
+    g, kw = _case(_abi.F64)
+    o = OracleCase(g.flat(), g.dims, **kw)
+    o.step(STEPS)
+    o.compute_strain()
+    lab = o.field("label")
+    out = {k: o.field(k) for k in ("rho", "vx", "vy", "vz", "sum_ruy")}
+    out["gamma"] = _gamma(o)
+    out["label"] = lab
+    n = STEPS  # statistics every step
+    lpa = -out["sum_ruy"][lab == 5].sum() / n
+    rpa = out["sum_ruy"][lab == 6].sum() / n
+    out["split"] = rpa / (lpa + rpa)
+    return out
+
+
+@pytest.mark.parametrize("storage", [_abi.F32, _abi.F16_OFFSET, _abi.BF16_OFFSET])
+def test_storage_against_fp64_oracle(gpu_lib, reference, storage):
+    g, kw = _case(storage)
+    c = Case(g.flat(), g.dims, **kw)
+    c.step(STEPS)
+    c.compute_strain()
+    act = reference["label"] != 1
+    u = np.stack([c.field(k)[act] for k in ("vx", "vy", "vz")])
+    uref = np.stack([reference[k][act] for k in ("vx", "vy", "vz")])
+    err_u = rel_l2(u, uref)
+    err_rho = rel_l2(c.field("rho")[act] - 1.0, reference["rho"][act] - 1.0)
+    err_g = rel_l2(_gamma(c)[act], reference["gamma"][act])
+    mf = c.flux(mean=True)
+    split = mf[3] / (mf[2] + mf[3])
+    err_split = 100.0 * abs(split - reference["split"])
+    name = _abi.STORAGE_NAMES[storage]
+    print(f"{name}: u {err_u:.3e} rho-1 {err_rho:.3e} gamma_dot {err_g:.3e} split {err_split:.5f} %")
+    tu, tr, tg = TOL[storage]
+    assert err_u < tu, (name, err_u)
+    assert err_rho < tr, (name, err_rho)
+    assert err_g < tg, (name, err_g)
+    assert err_split < 0.5, (name, err_split)
+    # the flow moved: the comparison is not of two rest states
+    assert np.abs(uref).max() > 0.01

@@ -472,6 +472,63 @@ def test_boundary_planes_closed_form():
     np.testing.assert_allclose(vx2[0, ys, zs], 0.04 * np.maximum(0.0, 1 - r2), rtol=1e-14, atol=1e-18)
 
 
+# inward normal of each boundary-plane label: the partner cell whose incoming
+# populations a BC cell pulls sits one plane inwards (sim_tcpc.cu:242-273)
+_INWARD = {3: (1, 0, 0), 4: (-1, 0, 0), 5: (0, 1, 0), 6: (0, -1, 0), 7: (-1, 0, 0), 8: (1, 0, 0)}
+_OUTFLOW_FACES = (_abi.FACE_OUTFLOW,) * 4 + (_abi.FACE_WALL,) * 2
+
+
+@pytest.mark.parametrize("faces", [(0,) * 6, _OUTFLOW_FACES, _abi.CHANNEL_FACES], ids=["tcpc", "outflows", "channel"])
+def test_boundary_partner_reads_tagged(faces):
+    """The non-local reads of the boundary planes, pinned by tagged populations.
+
+    Restated independently of the oracle's index arrays: a cell C of label L
+    pulls, for each direction q, the population f_q that streams into its
+    partner P = C + n_L (n_L the plane's inward normal) from P - c_q, every
+    coordinate clamped into the grid (sim_tcpc.cu:246,256,263,269 with the
+    clamped neighbour ids of core.h:651-661).  Inflow cells (3/4) then carry
+    rho = sum_q f_q; outflow cells (5/6 and the x outflows 7/8) carry
+    u = sum_q c_q f_q / rho and rho = 1.  Every population holds a distinct
+    random integer below 2^20, so the sums are exact and a read from any
+    other cell shows as a different value; the edge cells where the clamp
+    changes a source are counted and must be present."""
+    dims = (9, 8, 7)
+    X, Y, Z = dims
+    mask = np.zeros(dims, dtype=np.int32)
+    mask[4:6, 3:5, 2:4] = 1   # an interior wall block: some sources hold bounce-back slots
+    kw = dict(storage=_abi.F64, collision=_abi.CUM, nu_ref=0.05, face_bc=faces, dl=1e-3, dt=1e-3,
+              ivc_flux=1.0, svc_flux=1.0)
+    case = O.OracleCase(mask.transpose(2, 1, 0).ravel(), dims, **kw)
+    lab = case.field("label").astype(int)
+    ids = RNG.choice(np.arange(1, 1 << 20), size=27 * X * Y * Z, replace=False).astype(np.float64)
+    ids = ids.reshape(27, *dims)
+    case.set_populations(ids)
+    case.step(1)
+    rho, ux, uy, uz = (case.field(k) for k in ("rho", "vx", "vy", "vz"))
+    hi = np.array(dims) - 1
+    checked = {L: 0 for L in _INWARD}
+    clamped = 0
+    for C in zip(*np.nonzero(np.isin(lab, list(_INWARD)))):
+        L = int(lab[C])
+        P = np.array(C) + np.array(_INWARD[L])
+        src = P[None, :] - CV
+        clamped += bool(np.any((src < 0) | (src > hi)))
+        src = np.clip(src, 0, hi)
+        f = ids[Q, src[:, 0], src[:, 1], src[:, 2]]
+        r, J = f.sum(), CV.T @ f
+        if L in (3, 4):
+            assert rho[C] == r, (C, L)
+        else:
+            assert rho[C] == 1.0, (C, L)
+            assert (ux[C], uy[C], uz[C]) == tuple(J / r), (C, L)
+        checked[L] += 1
+    present = {L for L, n in checked.items() if n}
+    expect = {(0,) * 6: {3, 4, 5, 6}, _OUTFLOW_FACES: {5, 6, 7, 8}, _abi.CHANNEL_FACES: {3, 7}}[faces]
+    assert present == expect
+    # y-plane cells on the x faces (tcpc, outflows) have an x-clamped source
+    assert clamped > 0 or faces == _abi.CHANNEL_FACES
+
+
 @pytest.mark.slow
 def test_steady_mass_balance():
     """IVC + SVC inflow equals the label-5/6 outflow once the flow is steady."""
