@@ -14,7 +14,7 @@ from nonnewlbm_amd import _abi, flows, run_all
 from nonnewlbm_amd.geometry import Geometry, read_wall_vtk, tcpc_mask, write_wall_vtk
 from nonnewlbm_amd.lbm355 import Case, labels
 from nonnewlbm_amd.materials import case_kwargs, material_by_name
-from nonnewlbm_amd.run_case import CONFIGS, load_yaml
+from nonnewlbm_amd.run_case import study
 
 pytestmark = pytest.mark.gpu
 
@@ -45,7 +45,7 @@ def test_run_all_pool_summary_equals_cases_alone(gpu_lib, tmp_path):
     rows = _summary(tmp_path)
     assert [r["case_id"] for r in rows] == ["v1_newtonian", "v1_sx", "v2_newtonian", "v2_sx"]
     g = read_wall_vtk(geo)
-    regimes = load_yaml(CONFIGS / "regimes.yaml")["regimes"]
+    regimes = study().regimes
     for r in rows:
         assert r["state"] == "COMPLETED" and r["harvest_status"] == "ok" and r["source"] == "device_flux"
         kw = case_kwargs(material_by_name(r["material"]), g.dl, 1e-3)

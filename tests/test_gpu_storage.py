@@ -9,10 +9,13 @@ over the non-wall cells of velocity, of rho - 1, of the shear rate of the
 output path (FD strain, core.h:407-576) and the LPA share of the mean outlet
 flow (compute_flows.py:141-147 on the label-5/6 planes).
 
-Tolerances (about twice the errors measured on the MI355X): FP32 velocity
-1e-4; FP16-offset velocity 1e-3 and shear rate 2e-3; BF16-offset velocity
-5e-3 (it is refused for serving); the flow split within 0.5 % (in share
-points) for every type.
+Measured on the MI355X (velocity / rho - 1 / shear rate / split points):
+FP32 2.2e-6 / 2.4e-6 / 2.6e-6 / 0.00000; FP16-offset 2.4e-4 / 1.4e-4 / 3.0e-4 /
+0.00007; BF16-offset 1.9e-3 / 1.1e-3 / 2.4e-3 / 0.0013.  Tolerances: FP32
+1e-4 on each field; FP16-offset 1e-3 on velocity and
+rho - 1, 2e-3 on the shear rate; BF16-offset 5e-3 / 5e-3 / 1e-2 (it is
+refused for serving); the flow split within 0.5 share points for every
+type.
 """
 from __future__ import annotations
 
