@@ -51,14 +51,14 @@ def profiled_traffic(name: str) -> int | None:
     return None
 
 
-def make_case(scale: float, storage: int, collision: int, lbm_viscosity: float):
+def make_case(scale: float, storage: int, collision: int, lbm_viscosity: float, cy_model: int | None = None):
     from nonnewlbm_amd import _abi
     from nonnewlbm_amd.geometry import tcpc_mask
     from nonnewlbm_amd.materials import case_kwargs, material_by_name
 
     g = tcpc_mask(scale)
     mat = material_by_name("cy-a")
-    kw = case_kwargs(mat, g.dl, lbm_viscosity)
+    kw = case_kwargs(mat, g.dl, lbm_viscosity, cy_model=cy_model)
     kw.update(storage=storage, collision=collision, profile=_abi.PROFILE_PARABOLIC, ivc_umax=0.04,
               svc_umax=0.03, ivc_center=g.meta["ivc"]["center"], svc_center=g.meta["svc"]["center"],
               ivc_radius=g.meta["ivc"]["radius"], svc_radius=g.meta["svc"]["radius"], stats_every=1)
@@ -455,6 +455,8 @@ def main() -> int:
     ap.add_argument("--storage", default="fp32")
     ap.add_argument("--collision", default="cumulant")
     ap.add_argument("--lbm-viscosity", type=float, default=0.01)
+    ap.add_argument("--cy-model", default="neq", choices=["neq", "fd-force", "off"],
+                    help="Carreau-Yasuda coupling of the workload (fd-force adds the reference's pre-kernels)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-hbm", action="store_true", help="skip the 512x384x256 HBM-resident line")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -491,7 +493,8 @@ def main() -> int:
     collision = _abi.COLLISIONS[args.collision]
     sbytes = _abi.STORAGE_BYTES[storage]
 
-    g, kw = make_case(1.0, storage, collision, args.lbm_viscosity)
+    cy_model = {"neq": _abi.CY_NEQ, "fd-force": _abi.CY_FD_FORCE, "off": _abi.CY_OFF}[args.cy_model]
+    g, kw = make_case(1.0, storage, collision, args.lbm_viscosity, cy_model)
     case = Case(g.flat(), g.dims, **kw)
     info = case.info()
     active = info["active_cells"]
@@ -542,7 +545,8 @@ def main() -> int:
     print(f"# k_step {step_ms * 1e3:.2f} us per launch back to back -> {achieved:.0f} GB/s algorithmic "
           f"({kern_ms * 1e3:.2f} us with an event pair per launch); copy {copy_gbs:.0f} GB/s; finite={finite}",
           file=sys.stderr, flush=True)
-    default_workload = (storage == _abi.F32 and collision == _abi.CUM and args.lbm_viscosity == 0.01)
+    default_workload = (storage == _abi.F32 and collision == _abi.CUM and args.lbm_viscosity == 0.01
+                        and cy_model == _abi.CY_NEQ)
     traffic = profiled_traffic("k_step_tcpc256_cum_f32_cy_stats") if default_workload else None
 
     out = {
@@ -593,7 +597,7 @@ def main() -> int:
     }
 
     if not args.no_hbm:
-        g2, kw2 = make_case(2.0, storage, collision, args.lbm_viscosity)
+        g2, kw2 = make_case(2.0, storage, collision, args.lbm_viscosity, cy_model)
         c2 = Case(g2.flat(), g2.dims, **kw2)
         i2 = c2.info()
         bpg2, nt2, cap2 = configure(c2, args.launch_hbm)
