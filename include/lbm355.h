@@ -278,14 +278,34 @@ int lbm355_case_load_state(lbm355_case* c, const char* dir, lbm355_state_desc* s
 /* No counterpart in the reference (it runs one case per process,
    scripts/slurm_run_cases.py:104-198).  A batch steps cases that share
    collision and storage in one fused launch over the concatenated brick
-   index; graph=1 replays a captured hipGraph of the step. */
+   index, kept with the members' constants in device memory; graph=1 replays
+   one captured hipGraph of 64 steps per group.  A member keeps its own step
+   count and can be stepped alone between batch calls (the batch orders
+   itself after the work queued on the member's stream, and the member's
+   later work after the batch's). */
 typedef struct lbm355_batch lbm355_batch;
 int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batch** out);
 int lbm355_batch_step(lbm355_batch* b, int64_t n);
 int lbm355_batch_destroy(lbm355_batch* b);
+/* membership changes between batch calls: the device brick list and member
+   constants are rebuilt and re-uploaded; the captured graph stays valid
+   unless the largest group outgrows its launch grid.  EINVAL for a case of
+   another collision/storage type, an fd-force case or a member already in;
+   ENOTFOUND for removing a non-member.  A batch may become empty. */
+int lbm355_batch_add(lbm355_batch* b, lbm355_case* c);
+int lbm355_batch_remove(lbm355_batch* b, lbm355_case* c);
+int lbm355_batch_size(const lbm355_batch* b, int* members, int64_t* bricks);
 /* launch shape: 64-cell bricks (wavefronts) per workgroup, 1, 2, 4 or 8 */
 int lbm355_batch_set_shape(lbm355_batch* b, int bricks_per_group);
 int lbm355_batch_set_occupancy(lbm355_batch* b, int waves_per_cu);
+/* bricks per wavefront: 1 (default) launches one wavefront per brick; k > 1
+   launches the persistent form, a grid of about 1/k of that whose wavefronts
+   loop over the group's bricks */
+int lbm355_batch_set_per_wave(lbm355_batch* b, int per_wave);
+/* out = {bricks per workgroup, non-temporal (-1 auto), wavefront cap, group
+   MiB (0 = one group), bricks per wavefront, groups, membership uploads,
+   graph captures} */
+int lbm355_batch_config(const lbm355_batch* b, int out[8]);
 /* members are stepped in groups of consecutive cases holding at most `bytes`
    of device memory (a larger case is a group of its own; 0 = one group); each
    group runs a whole chunk of steps before the next, so a group that fits the
@@ -295,12 +315,13 @@ int lbm355_batch_set_group_bytes(lbm355_batch* b, int64_t bytes);
 /* times every launch shape (1, 2, 4, 8 bricks per workgroup), with and
    without the non-temporal even step, and with no cap, 16 or 12 wavefronts per
    CU, on real steps: 64 warm-up steps, then per configuration 2 warm-up steps
-   and a timed window W of 16 x stats_every steps (a whole number of statistics
-   groups; 8 without statistics, at most 128), then one group, 256 MiB and
-   128 MiB groups on 2 + 2W steps each.  All of them count as simulation steps
-   of every member (64 + 24 (2 + W) + 3 (2 + 2W)).  Keeps the fastest;
+   (plus 64 that capture the graph when the window reaches 64 steps) and a
+   timed window W of 16 x stats_every steps (a whole number of statistics
+   groups; 8 without statistics, at most 128); then 2 and 4 bricks per
+   wavefront, then one group, 256 MiB and 128 MiB groups on windows of 2W.
+   All of them count as simulation steps of every member.  Keeps the fastest;
    *best = bricks_per_group | nontemporal << 8 | waves_per_cu << 16 |
-           (group bytes >> 24) << 24 */
+           (group bytes >> 24) << 24 (lbm355_batch_config gives the rest) */
 int lbm355_batch_autotune(lbm355_batch* b, int* best);
 /* the same for one case without grouping; instead the 3 fastest
    configurations are timed again, in two interleaved passes of 2 + 2W steps,

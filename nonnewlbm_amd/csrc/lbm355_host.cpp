@@ -1186,47 +1186,18 @@ int lbm355_case_compute_strain(lbm355_case* c)
 int lbm355_case_flux(lbm355_case* c, int mean, double out[4])
 {
     if (!c || !out) return fail(LBM355_EINVAL, "null argument");
-    if (!mean) {
-        LAUNCH_TRY(launch_zero(c->flux.p, 32, c->stream));
-        LAUNCH_TRY(launch_flux(c->storage, static_cast<const CaseDev*>(c->devcase.p), c->n_bc,
-                               static_cast<double*>(c->flux.p), c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        HIP_TRY(hipMemcpy(out, c->flux.p, 32, hipMemcpyDeviceToHost));
-        return LBM355_OK;
-    }
-    // mean flux from the statistics sums of the BC cells (density weighted,
-    // as the mean_velocity output, sim_tcpc.cu:724-727)
-    if (c->d.stats_every <= 0) return fail(LBM355_EINVAL, "statistics are off");
-    const int na = c->n_active;
-    std::vector<double> h(size_t(std::max(na, 1)) * kStatFields * kLanes);
+    if (mean && c->d.stats_every <= 0) return fail(LBM355_EINVAL, "statistics are off");
+    // samples of the open group are still in the ring, not in the FP64 sums
+    const int open = mean && c->storage != LBM355_F64 ? int((c->stats_steps - c->stats_block0) % kStatBlock) : 0;
+    LAUNCH_TRY(launch_zero(c->flux.p, 32, c->stream));
+    LAUNCH_TRY(launch_flux(c->storage, static_cast<const CaseDev*>(c->devcase.p), c->n_bc,
+                           static_cast<double*>(c->flux.p), mean, open, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (na > 0) HIP_TRY(hipMemcpy(h.data(), c->stats.p, h.size() * 8, hipMemcpyDeviceToHost));
-    // samples of the open group, from the ring (only the first moments are used here)
-    const int64_t open = c->storage == LBM355_F64 ? 0 : (c->stats_steps - c->stats_block0) % kStatBlock;
-    std::vector<float> part(size_t(std::max(na, 1)) * kRingFields * kLanes);
-    for (int64_t sl = 0; sl < open; ++sl) {
-        if (na > 0) HIP_TRY(hipMemcpy(part.data(), ring_slot(*c, sl), part.size() * 4, hipMemcpyDeviceToHost));
-        for (int b = 0; b < na; ++b)
-            for (int a = 0; a < kRingFields; ++a)
-                for (int l = 0; l < kLanes; ++l)
-                    h[(size_t(b) * kStatFields + a) * kLanes + l] += double(part[(size_t(b) * kRingFields + a) * kLanes + l]);
-    }
-    const double diff = std::max<double>(1.0, double(c->stats_steps));
-    for (int p = 0; p < 4; ++p) out[p] = 0.0;
-    for (int s = 0; s < c->n_bc; ++s) {
-        const int cell = c->bc_cell[s], b = cell / kLanes, l = cell % kLanes;
-        const int lab = c->bc_label_h[s];
-        int plane, comp;
-        double sign;
-        switch (lab) {
-        case L_IN_XM: plane = 0; comp = 0; sign = 1.0; break;
-        case L_OUT_XM: plane = 0; comp = 0; sign = -1.0; break;
-        case L_IN_XP: plane = 1; comp = 0; sign = -1.0; break;
-        case L_OUT_XP: plane = 1; comp = 0; sign = 1.0; break;
-        case L_OUT_YM: plane = 2; comp = 1; sign = -1.0; break;
-        default: plane = 3; comp = 1; sign = 1.0; break;
-        }
-        out[plane] += sign * h[(size_t(b) * kStatFields + comp) * kLanes + l] / diff;
+    HIP_TRY(hipMemcpy(out, c->flux.p, 32, hipMemcpyDeviceToHost));
+    if (mean) {
+        // density weighted, as the mean_velocity output (sim_tcpc.cu:724-727)
+        const double diff = std::max<double>(1.0, double(c->stats_steps));
+        for (int p = 0; p < 4; ++p) out[p] /= diff;
     }
     return LBM355_OK;
 }
@@ -1649,7 +1620,12 @@ int lbm355_case_write_vtk(lbm355_case* c, const char* path, const lbm355_output_
 }
 
 // ---------------------------------------------------------------------------
-// batches
+// batches: several cases of one collision and storage type stepped by one
+// launch per step over their concatenated brick lists.  The list, the
+// members' constants and the brick range of the group being stepped live in
+// device memory (BatchDev), read by the batch kernels at run time: a
+// membership change re-uploads them and leaves a captured graph valid unless
+// the launch grid has to grow.
 struct lbm355_batch {
     std::vector<lbm355_case*> cases;
     int storage = 1, model = 0;
@@ -1657,13 +1633,18 @@ struct lbm355_batch {
     int bricks_per_group = 4;
     int nontemporal = -1;        // -1 auto (by the members' working set), 0 off, 1 on
     int waves_cap = 0;           // wavefronts per CU of the fused step, 0 = no cap
+    int per_wave = 1;            // bricks per wavefront: 1 one each (k_step_range), k > 1 the persistent loop
     hipStream_t stream = nullptr;
-    DevBuf devcases, list, base;
+    // recorded on the batch's stream after its steps; every member's stream
+    // waits on it, so whatever is queued for a case later runs after them
+    hipEvent_t done = nullptr;
+    DevBuf devcases, list, base, desc;
+    int64_t cap_cases = 0, cap_bricks = 0;  // capacity of devcases / list (elements)
     int n_list = 0;
     // the cases are stepped in groups of consecutive members whose device
-    // memory stays below group_bytes (0: one group); each group runs a whole
-    // chunk of steps before the next, so a group that fits the Infinity Cache
-    // stays there for the chunk
+    // memory a step sweeps stays below group_bytes (0: one group); each group
+    // runs a whole chunk of steps before the next, so a group that fits the
+    // Infinity Cache stays there for the chunk
     struct Group {
         int begin, end;   // brick list range
         int64_t bytes;
@@ -1672,17 +1653,15 @@ struct lbm355_batch {
     std::vector<int> case_first;        // brick list offset of each member
     std::vector<int64_t> case_bytes;    // device memory a step of each member sweeps (case_step_bytes)
     int64_t group_bytes = kBatchGroupBytes;
-    int64_t launches = 0;     // steps done by the batch
-    int64_t base_host = 0;    // mirror of the device launch base
-    std::vector<int64_t> start;  // per case: launch index of its step 0
+    int64_t launches = 0;     // steps done by the batch; the device launch counter equals it between calls
+    int cur_begin = -1, cur_end = -1;   // range last written to the device descriptor
+    std::vector<int64_t> start;  // per case: launch index of its step 0 in the batch copy
     std::vector<int64_t> epoch;  // per case: statistics epoch in the batch copy
     std::vector<double> in_xm, in_xp;  // per case: inflow scales in the batch copy
-    // recorded on the batch's stream after its steps; every member's stream
-    // waits on it, so whatever is queued for a case later runs after them
-    hipEvent_t done = nullptr;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
-    int graph_steps = 0;
+    int graph_grid = 0;          // launch grid baked into the graph (workgroups)
+    int64_t uploads = 0, captures = 0;  // membership uploads and graph captures so far
     ~lbm355_batch()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -1733,30 +1712,125 @@ void regroup(lbm355_batch& b)
     }
 }
 
-// one step of group g of a batch
-StepLaunch batch_launch(lbm355_batch& b, size_t g)
-{
-    const lbm355_batch::Group& G = b.groups[g];
-    StepLaunch L{};
-    L.list = static_cast<const BrickRef*>(b.list.p) + G.begin;
-    L.n_list = G.end - G.begin;
-    L.cases = static_cast<CaseDev*>(b.devcases.p);
-    L.n_cases = int32_t(b.cases.size());
-    L.base = static_cast<const int64_t*>(b.base.p);
-    L.rel = b.launches - b.base_host;
-    L.bricks_per_group = b.bricks_per_group;
-    L.nontemporal = b.nontemporal >= 0 ? b.nontemporal : (G.bytes > kNontemporalBytes ? 1 : 0);
-    L.lds_bytes = occupancy_lds(b.waves_cap, b.bricks_per_group);
-    return L;
-}
-
 void drop_graph(lbm355_batch& b)
 {
     if (b.exec) (void)hipGraphExecDestroy(b.exec);
     if (b.graph) (void)hipGraphDestroy(b.graph);
     b.exec = nullptr;
     b.graph = nullptr;
-    b.graph_steps = 0;
+    b.graph_grid = 0;
+}
+
+// launch grid (workgroups) that covers the largest group: one workgroup of
+// bricks_per_group bricks each, or per_wave bricks per wavefront for the
+// persistent loop; a multiple of 8 for the XCD split
+int batch_grid(const lbm355_batch& b)
+{
+    int most = 0;
+    for (const auto& g : b.groups) most = std::max(most, g.end - g.begin);
+    int blocks = (most + b.bricks_per_group - 1) / b.bricks_per_group;
+    if (b.per_wave > 1) blocks = (blocks + b.per_wave - 1) / b.per_wave;
+    return std::max(8, (blocks + 7) / 8 * 8);
+}
+
+StepLaunch batch_step_launch(lbm355_batch& b, int grid)
+{
+    StepLaunch L{};
+    L.batch = static_cast<const BatchDev*>(b.desc.p);
+    L.n_list = grid;
+    L.base = static_cast<const int64_t*>(b.base.p);
+    L.rel = 0;
+    L.bricks_per_group = b.bricks_per_group;
+    int64_t most = 0;
+    for (const auto& g : b.groups) most = std::max(most, g.bytes);
+    L.nontemporal = b.nontemporal >= 0 ? b.nontemporal : (most > kNontemporalBytes ? 1 : 0);
+    L.lds_bytes = occupancy_lds(b.waves_cap, b.bricks_per_group);
+    L.per_wave = b.per_wave;
+    return L;
+}
+
+// point the device descriptor at brick range [begin, end) (a kernel on the
+// batch's stream: it orders with the steps around it and records no memset)
+int set_range(lbm355_batch& b, int begin, int end)
+{
+    if (begin == b.cur_begin && end == b.cur_end) return LBM355_OK;
+    BatchDev v;
+    v.list = static_cast<const BrickRef*>(b.list.p);
+    v.cases = static_cast<CaseDev*>(b.devcases.p);
+    v.begin = begin;
+    v.end = end;
+    LAUNCH_TRY(launch_set_batch(static_cast<BatchDev*>(b.desc.p), v, b.stream));
+    b.cur_begin = begin;
+    b.cur_end = end;
+    return LBM355_OK;
+}
+
+// grow a device array to hold `need` elements of `elem` bytes (contents are
+// re-uploaded by the caller)
+int reserve(DevBuf& buf, int64_t& cap, int64_t need, size_t elem)
+{
+    if (need <= cap && buf.p) return LBM355_OK;
+    const int64_t n = std::max<int64_t>({need + need / 2, 16, cap});
+    DevBuf fresh;
+    if (fresh.alloc(size_t(n) * elem) != 0) return fail(LBM355_ENOMEM, "batch arrays");
+    std::swap(buf.p, fresh.p);
+    std::swap(buf.bytes, fresh.bytes);
+    cap = n;
+    return LBM355_OK;
+}
+
+// rebuild the brick list, the member constants and the groups after a
+// membership change and upload them; the batch's stream is idle.  A member's
+// step 0 sits at launch index launches - steps, so its own step count goes on.
+int batch_upload(lbm355_batch& b)
+{
+    std::vector<CaseDev> cd;
+    std::vector<BrickRef> list;
+    b.start.clear(); b.epoch.clear(); b.in_xm.clear(); b.in_xp.clear();
+    b.case_first.clear(); b.case_bytes.clear();
+    for (size_t i = 0; i < b.cases.size(); ++i) {
+        lbm355_case* c = b.cases[i];
+        cd.push_back(c->host);
+        CaseDev& h = cd.back();
+        h.start = b.launches - c->steps;
+        h.stats_epoch = c->stats_first;
+        h.macro_step = -1;  // macro output only on explicit request inside a batch
+        b.start.push_back(h.start);
+        b.epoch.push_back(h.stats_epoch);
+        b.in_xm.push_back(h.inflow_xm);
+        b.in_xp.push_back(h.inflow_xp);
+        b.case_first.push_back(int(list.size()));
+        b.case_bytes.push_back(case_step_bytes(*c));
+        for (int k = 0; k < c->n_active; ++k) list.push_back(BrickRef{uint32_t(i), uint32_t(k)});
+    }
+    // the cases' slab-ordered brick runs follow each other; the
+    // XCD-contiguous split of a group's range gives each XCD consecutive bricks
+    b.n_list = int(list.size());
+    regroup(b);
+    int rc;
+    if ((rc = reserve(b.devcases, b.cap_cases, int64_t(cd.size()), sizeof(CaseDev)))) return rc;
+    if ((rc = reserve(b.list, b.cap_bricks, int64_t(list.size()), sizeof(BrickRef)))) return rc;
+    if (!cd.empty()) HIP_TRY(hipMemcpy(b.devcases.p, cd.data(), cd.size() * sizeof(CaseDev), hipMemcpyHostToDevice));
+    if (!list.empty())
+        HIP_TRY(hipMemcpy(b.list.p, list.data(), list.size() * sizeof(BrickRef), hipMemcpyHostToDevice));
+    b.cur_begin = b.cur_end = -1;  // the descriptor's pointers may have moved
+    if ((rc = set_range(b, 0, 0))) return rc;
+    HIP_TRY(hipStreamSynchronize(b.stream));
+    // the graph bakes the grid; a larger group needs a larger one
+    if (b.exec && batch_grid(b) > b.graph_grid) drop_graph(b);
+    ++b.uploads;
+    return LBM355_OK;
+}
+
+int check_member(const lbm355_batch& b, const lbm355_case* c)
+{
+    if (!c) return fail(LBM355_EINVAL, "null case in batch");
+    if (c->storage != b.storage || c->model != b.model)
+        return fail(LBM355_EINVAL, "a batch needs one collision and one storage type");
+    if (c->host.force) return fail(LBM355_EINVAL, "fd-force cases step alone (their pre-kernels are per case)");
+    if (std::find(b.cases.begin(), b.cases.end(), c) != b.cases.end())
+        return fail(LBM355_EINVAL, "case is already a member of the batch");
+    return LBM355_OK;
 }
 }  // namespace
 
@@ -1765,48 +1839,80 @@ int lbm355_batch_create(lbm355_case* const* cases, int n, int graph, lbm355_batc
     if (!cases || n <= 0 || !out) return fail(LBM355_EINVAL, "bad batch arguments");
     std::unique_ptr<lbm355_batch> b(new (std::nothrow) lbm355_batch());
     if (!b) return fail(LBM355_ENOMEM, "host allocation");
+    if (!cases[0]) return fail(LBM355_EINVAL, "null case in batch");
     b->storage = cases[0]->storage;
     b->model = cases[0]->model;
     b->use_graph = graph;
-    std::vector<CaseDev> cd;
-    std::vector<BrickRef> list;
+    int rc;
     for (int i = 0; i < n; ++i) {
-        lbm355_case* c = cases[i];
-        if (!c) return fail(LBM355_EINVAL, "null case in batch");
-        if (c->storage != b->storage || c->model != b->model)
-            return fail(LBM355_EINVAL, "a batch needs one collision and one storage type");
-        if (c->host.force)
-            return fail(LBM355_EINVAL, "fd-force cases step alone (their pre-kernels are per case)");
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        b->cases.push_back(c);
-        cd.push_back(c->host);
-        cd.back().start = -c->steps;  // batch launch 0 is the case's step c->steps
-        b->start.push_back(-c->steps);
-        b->epoch.push_back(c->stats_first);
-        b->in_xm.push_back(c->host.inflow_xm);
-        b->in_xp.push_back(c->host.inflow_xp);
-        b->case_first.push_back(int(list.size()));
-        b->case_bytes.push_back(case_step_bytes(*c));
-        for (int k = 0; k < c->n_active; ++k) list.push_back(BrickRef{uint32_t(i), uint32_t(k)});
+        if ((rc = check_member(*b, cases[i]))) return rc;
+        HIP_TRY(hipStreamSynchronize(cases[i]->stream));
+        b->cases.push_back(cases[i]);
     }
-    // the cases' slab-ordered brick runs follow each other; the
-    // XCD-contiguous split of a group's range gives each XCD consecutive bricks
-    b->n_list = int(list.size());
-    regroup(*b);
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
-    int rc;
-    if ((rc = upload(b->devcases, cd))) return rc;
-    if ((rc = upload(b->list, list))) return rc;
-    if (b->base.alloc(16) != 0) return fail(LBM355_ENOMEM, "launch counter");
+    if (b->base.alloc(16) != 0 || b->desc.alloc(sizeof(BatchDev)) != 0) return fail(LBM355_ENOMEM, "batch counters");
     LAUNCH_TRY(launch_zero(b->base.p, 16, b->stream));
-    // macro output only on explicit request inside a batch
-    for (int i = 0; i < n; ++i) {
-        CaseDev* dc = static_cast<CaseDev*>(b->devcases.p) + i;
-        LAUNCH_TRY(launch_set_u64(&dc->macro_step, uint64_t(-1), b->stream));
-    }
-    HIP_TRY(hipStreamSynchronize(b->stream));
+    if ((rc = batch_upload(*b))) return rc;
     *out = b.release();
+    return LBM355_OK;
+}
+
+int lbm355_batch_add(lbm355_batch* b, lbm355_case* c)
+{
+    if (!b) return fail(LBM355_EINVAL, "null batch");
+    int rc;
+    if ((rc = check_member(*b, c))) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    b->cases.push_back(c);
+    if ((rc = batch_upload(*b))) {
+        b->cases.pop_back();
+        return rc;
+    }
+    return LBM355_OK;
+}
+
+int lbm355_batch_remove(lbm355_batch* b, lbm355_case* c)
+{
+    if (!b || !c) return fail(LBM355_EINVAL, "null argument");
+    auto it = std::find(b->cases.begin(), b->cases.end(), c);
+    if (it == b->cases.end()) return fail(LBM355_ENOTFOUND, "case is not a member of the batch");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->cases.erase(it);
+    return batch_upload(*b);
+}
+
+}  // extern "C"
+
+// several membership changes with one upload (the pool retires and admits
+// cases between ticks)
+int batch_update(lbm355_batch* b, const std::vector<lbm355_case*>& add, const std::vector<lbm355_case*>& remove)
+{
+    if (!b) return fail(LBM355_EINVAL, "null batch");
+    if (add.empty() && remove.empty()) return LBM355_OK;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (lbm355_case* c : remove) {
+        auto it = std::find(b->cases.begin(), b->cases.end(), c);
+        if (it == b->cases.end()) return fail(LBM355_ENOTFOUND, "case is not a member of the batch");
+        b->cases.erase(it);
+    }
+    for (lbm355_case* c : add) {
+        int rc;
+        if ((rc = check_member(*b, c))) return rc;
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        b->cases.push_back(c);
+    }
+    return batch_upload(*b);
+}
+
+extern "C" {
+
+int lbm355_batch_size(const lbm355_batch* b, int* members, int64_t* bricks)
+{
+    if (!b) return fail(LBM355_EINVAL, "null batch");
+    if (members) *members = int(b->cases.size());
+    if (bricks) *bricks = b->n_list;
     return LBM355_OK;
 }
 
@@ -1814,6 +1920,14 @@ int lbm355_batch_set_shape(lbm355_batch* b, int bpg)
 {
     if (!b || !(bpg == 1 || bpg == 2 || bpg == 4 || bpg == 8)) return fail(LBM355_EINVAL, "bricks per group");
     b->bricks_per_group = bpg;
+    drop_graph(*b);
+    return LBM355_OK;
+}
+
+int lbm355_batch_set_per_wave(lbm355_batch* b, int per_wave)
+{
+    if (!b || per_wave < 1 || per_wave > 64) return fail(LBM355_EINVAL, "bricks per wavefront is 1..64");
+    b->per_wave = per_wave;
     drop_graph(*b);
     return LBM355_OK;
 }
@@ -1836,6 +1950,20 @@ int lbm355_batch_set_occupancy(lbm355_batch* b, int waves_per_cu)
     return LBM355_OK;
 }
 
+int lbm355_batch_config(const lbm355_batch* b, int out[8])
+{
+    if (!b || !out) return fail(LBM355_EINVAL, "null argument");
+    out[0] = b->bricks_per_group;
+    out[1] = b->nontemporal;
+    out[2] = b->waves_cap;
+    out[3] = int(b->group_bytes >> 20);
+    out[4] = b->per_wave;
+    out[5] = int(b->groups.size());
+    out[6] = int(b->uploads);
+    out[7] = int(b->captures);
+    return LBM355_OK;
+}
+
 }  // extern "C"
 
 // the library's error channel for the other host sources
@@ -1848,7 +1976,7 @@ int64_t lbm355_case_step_bytes(const lbm355_case* c) { return c ? case_step_byte
 int batch_enqueue(lbm355_batch* b, int64_t n)
 {
     if (!b) return fail(LBM355_EINVAL, "null batch");
-    if (n <= 0) return LBM355_OK;
+    if (n <= 0 || b->cases.empty()) return LBM355_OK;
     // the batch's steps follow whatever is queued on each member's own stream
     // (steps, statistics resets, inflow updates): the streams are
     // independent, so without this wait both could touch one case at once
@@ -1886,22 +2014,23 @@ int batch_enqueue(lbm355_batch* b, int64_t n)
         // rho,u of every case at the last step of this call
         LAUNCH_TRY(launch_set_u64(&dc->macro_step, uint64_t(b->cases[i]->steps + n - 1), b->stream));
     }
+    // steps per group per replay: long enough that refilling the Infinity
+    // Cache at a group switch is a few percent of the chunk
+    const int chunk = 64;
+    const int grid = std::max(batch_grid(*b), b->graph_grid);
     int64_t done = 0;
-    if (b->use_graph) {
-        // steps per group per replay: long enough that refilling the
-        // Infinity Cache at a group switch is a few percent of the chunk
-        const int chunk = 64;
+    if (b->use_graph && n >= chunk) {
         if (!b->exec) {
+            // one graph of `chunk` consecutive launches (launch index base +
+            // 0..chunk-1) of the group whose range the descriptor holds;
+            // replayed once per group, then the launch counter advances
+            StepLaunch L = batch_step_launch(*b, grid);
             HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             int r = 0;
-            for (size_t g = 0; g < b->groups.size() && r == 0; ++g) {
-                StepLaunch L = batch_launch(*b, g);
-                for (int s = 0; s < chunk && r == 0; ++s) {
-                    L.rel = s;
-                    r = launch_step(b->storage, b->model, L, b->stream);
-                }
+            for (int s = 0; s < chunk && r == 0; ++s) {
+                L.rel = s;
+                r = launch_step(b->storage, b->model, L, b->stream);
             }
-            if (r == 0) r = launch_advance(static_cast<int64_t*>(b->base.p), chunk, b->stream);
             hipGraph_t g = nullptr;
             const hipError_t e = hipStreamEndCapture(b->stream, &g);
             if (r != 0 || e != hipSuccess) {
@@ -1910,31 +2039,33 @@ int batch_enqueue(lbm355_batch* b, int64_t n)
             }
             b->graph = g;
             HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-            b->graph_steps = chunk;
+            b->graph_grid = grid;
+            ++b->captures;
         }
-        if (n >= b->graph_steps && b->base_host != b->launches) {
-            // host-driven steps since the last replay: move the device base up
-            LAUNCH_TRY(launch_advance(static_cast<int64_t*>(b->base.p), b->launches - b->base_host, b->stream));
-            b->base_host = b->launches;
-        }
-        while (n - done >= b->graph_steps) {
-            HIP_TRY(hipGraphLaunch(b->exec, b->stream));
-            done += b->graph_steps;
-            b->launches += b->graph_steps;
-            b->base_host += b->graph_steps;
+        while (n - done >= chunk) {
+            int rc;
+            for (const auto& G : b->groups) {
+                if ((rc = set_range(*b, G.begin, G.end))) return rc;
+                HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+            }
+            LAUNCH_TRY(launch_advance(static_cast<int64_t*>(b->base.p), chunk, b->stream));
+            done += chunk;
+            b->launches += chunk;
         }
     }
     // the remaining steps group by group; every group replays the same
     // launch indices, so each member sees its own consecutive steps
     if (n > done) {
-        for (size_t g = 0; g < b->groups.size(); ++g) {
-            StepLaunch L = batch_launch(*b, g);
-            const int64_t rel0 = L.rel;
+        StepLaunch L = batch_step_launch(*b, grid);
+        int rc;
+        for (const auto& G : b->groups) {
+            if ((rc = set_range(*b, G.begin, G.end))) return rc;
             for (int64_t s = 0; s < n - done; ++s) {
-                L.rel = rel0 + s;
+                L.rel = s;
                 LAUNCH_TRY(launch_step(b->storage, b->model, L, b->stream));
             }
         }
+        LAUNCH_TRY(launch_advance(static_cast<int64_t*>(b->base.p), n - done, b->stream));
         b->launches += n - done;
     }
     // and whatever is queued for a member afterwards follows the batch's steps
@@ -1970,70 +2101,79 @@ int lbm355_batch_step(lbm355_batch* b, int64_t n)
     return n > 0 ? batch_finish(b) : LBM355_OK;
 }
 
+namespace {
+// wall time of `steps` batch steps after 2 warm-up steps (which also capture
+// the graph), ms
+int time_batch(lbm355_batch* b, int steps, hipEvent_t e0, hipEvent_t e1, float* ms)
+{
+    int rc;
+    if ((rc = batch_enqueue(b, 2)) || (rc = batch_finish(b))) return rc;
+    if (b->use_graph && steps >= 64 && (rc = batch_enqueue(b, 64))) return rc;  // graph capture outside the window
+    if ((rc = batch_finish(b))) return rc;
+    HIP_TRY(hipEventRecord(e0, b->stream));
+    if ((rc = batch_enqueue(b, steps))) return rc;
+    HIP_TRY(hipEventRecord(e1, b->stream));
+    if ((rc = batch_finish(b))) return rc;
+    HIP_TRY(hipEventElapsedTime(ms, e0, e1));
+    return LBM355_OK;
+}
+}  // namespace
+
 int lbm355_batch_autotune(lbm355_batch* b, int* best)
 {
     if (!b) return fail(LBM355_EINVAL, "null batch");
-    // real steps of the batch, an even number per configuration so both AA
-    // parities count; the fastest launch shape, cache policy and wavefront
-    // cap are kept
+    if (b->cases.empty()) return fail(LBM355_EINVAL, "empty batch");
+    // real steps of the batch, a whole number of statistics groups per window
+    // so both AA parities and the same number of flushes count; coordinate
+    // search: launch shape x cache policy x wavefront cap, then bricks per
+    // wavefront (the persistent loop), then the group size
     int steps = 8;
     for (const lbm355_case* c : b->cases) steps = std::max(steps, tune_window(c->d.stats_every));
-    int rc0 = batch_enqueue(b, kTuneWarmup);
-    if (rc0 || (rc0 = batch_finish(b))) return rc0;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    int rc = batch_enqueue(b, kTuneWarmup);
+    if (rc || (rc = batch_finish(b))) return rc;
+    EventSet es(2);
+    HIP_TRY(hipEventCreate(&es.ev[0]));
+    HIP_TRY(hipEventCreate(&es.ev[1]));
     float best_ms = 0.f;
-    int best_bpg = b->bricks_per_group, best_nt = b->nontemporal, best_cap = b->waves_cap;
-    int rc = LBM355_OK;
+    int best_bpg = b->bricks_per_group, best_nt = b->nontemporal, best_cap = b->waves_cap, best_pw = 1;
     bool first = true;
-    for (int k = 0; k < kTuneConfigs && rc == LBM355_OK; ++k) {
+    b->per_wave = 1;
+    for (int k = 0; k < kTuneConfigs; ++k) {
         const TuneConfig t = tune_config(k);
         b->bricks_per_group = t.bpg;
         b->nontemporal = t.nt;
         b->waves_cap = t.cap;
         drop_graph(*b);
-        if ((rc = batch_enqueue(b, 2)) || (rc = batch_finish(b))) break;  // warm-up (and graph capture)
-        if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
-        if ((rc = batch_enqueue(b, steps))) break;
-        if (hipEventRecord(e1, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
-        if ((rc = batch_finish(b))) break;
         float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
+        if ((rc = time_batch(b, steps, es.ev[0], es.ev[1], &ms))) return rc;
         if (first || ms < best_ms) { best_ms = ms; best_bpg = t.bpg; best_nt = t.nt; best_cap = t.cap; }
         first = false;
     }
-    if (rc == LBM355_OK) {
-        // then the grouping, with the launch configuration just chosen
-        b->bricks_per_group = best_bpg;
-        b->nontemporal = best_nt;
-        b->waves_cap = best_cap;
-        const int64_t sizes[3] = {0, int64_t(256) << 20, int64_t(128) << 20};
-        int64_t best_group = b->group_bytes;
-        bool gfirst = true;
-        for (int k = 0; k < 3 && rc == LBM355_OK; ++k) {
-            b->group_bytes = sizes[k];
-            regroup(*b);
-            drop_graph(*b);
-            if ((rc = batch_enqueue(b, 2)) || (rc = batch_finish(b))) break;
-            if (hipEventRecord(e0, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
-            if ((rc = batch_enqueue(b, 2 * steps))) break;
-            if (hipEventRecord(e1, b->stream) != hipSuccess) { rc = fail(LBM355_EDEVICE, "event"); break; }
-            if ((rc = batch_finish(b))) break;
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            if (gfirst || ms < best_ms) { best_ms = ms; best_group = sizes[k]; }
-            gfirst = false;
-        }
-        b->group_bytes = best_group;
-        regroup(*b);
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) return rc;
     b->bricks_per_group = best_bpg;
     b->nontemporal = best_nt;
     b->waves_cap = best_cap;
+    for (int pw : {2, 4}) {
+        b->per_wave = pw;
+        drop_graph(*b);
+        float ms = 0.f;
+        if ((rc = time_batch(b, steps, es.ev[0], es.ev[1], &ms))) return rc;
+        if (ms < best_ms) { best_ms = ms; best_pw = pw; }
+    }
+    b->per_wave = best_pw;
+    const int64_t sizes[3] = {0, int64_t(256) << 20, int64_t(128) << 20};
+    int64_t best_group = b->group_bytes;
+    bool gfirst = true;
+    for (int64_t gb : sizes) {
+        b->group_bytes = gb;
+        regroup(*b);
+        drop_graph(*b);
+        float ms = 0.f;
+        if ((rc = time_batch(b, 2 * steps, es.ev[0], es.ev[1], &ms))) return rc;
+        if (gfirst || ms < 2.f * best_ms) { best_ms = ms * 0.5f; best_group = gb; }
+        gfirst = false;
+    }
+    b->group_bytes = best_group;
+    regroup(*b);
     drop_graph(*b);
     if (best) *best = best_bpg | (best_nt << 8) | (best_cap << 16) | int(b->group_bytes >> 24) << 24;
     return LBM355_OK;

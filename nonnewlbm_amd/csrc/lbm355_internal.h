@@ -117,7 +117,19 @@ constexpr int64_t kNontemporalBytes = int64_t(448) << 20;
 // batches step their members in groups of at most this much device memory
 constexpr int64_t kBatchGroupBytes = int64_t(256) << 20;
 
-// launch descriptor of one fused step over a brick list
+// device-resident descriptor of a batch: its concatenated brick list, its
+// members' constants, and the brick range of the group the next persistent
+// launches step (changed between launches or graph replays, never inside)
+struct BatchDev {
+    const BrickRef* list;
+    CaseDev* cases;
+    int32_t begin, end;
+};
+
+// launch descriptor of one fused step over a brick list.  With batch == null
+// it is one brick per wavefront over list[0, n_list) (k_step); otherwise the
+// persistent form over the batch descriptor's range with a grid of n_list
+// workgroups (k_step_loop)
 struct StepLaunch {
     const BrickRef* list;
     int32_t n_list;
@@ -128,6 +140,9 @@ struct StepLaunch {
     int32_t bricks_per_group; // 1, 2, 4, 8 wavefronts per workgroup
     int32_t nontemporal;      // 1: even-step population accesses with the nt hint
     int32_t lds_bytes;        // dynamic LDS per workgroup (unused by the kernel): caps wavefronts per CU
+    const BatchDev* batch;    // batch launch (device descriptor), or null
+    int32_t per_wave;         // batch launches: 1 one brick per wavefront (k_step_range), k > 1 the
+                              // persistent loop (k_step_loop, grid sized for about k bricks per wavefront)
 };
 
 // storage kinds (same values as LBM355_F64 ...)
@@ -168,8 +183,11 @@ int launch_strain(int storage, const CaseDev* dev_case, int n_active, void* stre
 // fd-force Carreau-Yasuda mode: velocity pre-pass, strain and body force of
 // the step L is about to run (launch them right before it on the same stream)
 int launch_fd_force_prepass(int storage, const StepLaunch& L, const CaseDev* dev_case, int n_active, void* stream);
-int launch_flux(int storage, const CaseDev* dev_case, int n_bc, double* dev_out, void* stream);
+// plane fluxes of the BC cells into dev_out[4] (accumulated: clear it first);
+// mean = 1 sums the statistics (FP64 sums + `open` ring samples) instead of u
+int launch_flux(int storage, const CaseDev* dev_case, int n_bc, double* dev_out, int mean, int open, void* stream);
 int launch_set_u64(void* dev_ptr, uint64_t value, void* stream);
+int launch_set_batch(BatchDev* dev_ptr, const BatchDev& value, void* stream);
 int launch_zero(void* ptr, int64_t bytes, void* stream);
 const char* kernel_error();
 

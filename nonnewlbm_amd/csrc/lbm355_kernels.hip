@@ -604,25 +604,41 @@ template <class R>
 struct CyK;
 template <>
 struct CyK<double> {
-    __device__ static double nu_ref(const CaseDev& C) { return C.nu_ref; }
-    __device__ static double nu0(const CaseDev& C) { return C.cy_nu0; }
-    __device__ static double a(const CaseDev& C) { return C.cy_a; }
-    __device__ static double n(const CaseDev& C) { return C.cy_n; }
-    __device__ static double e(const CaseDev& C) { return C.cy_e; }
-    __device__ static double log2_lambda(const CaseDev& C) { return C.cy_log2_lambda; }
-    __device__ static double log2_tau_max(const CaseDev& C) { return C.cy_log2_tau_max; }
-    __device__ static double log2_tau_min(const CaseDev& C) { return C.cy_log2_tau_min; }
+    template <class CT>
+    __device__ static double nu_ref(const CT& C) { return C.nu_ref; }
+    template <class CT>
+    __device__ static double nu0(const CT& C) { return C.cy_nu0; }
+    template <class CT>
+    __device__ static double a(const CT& C) { return C.cy_a; }
+    template <class CT>
+    __device__ static double n(const CT& C) { return C.cy_n; }
+    template <class CT>
+    __device__ static double e(const CT& C) { return C.cy_e; }
+    template <class CT>
+    __device__ static double log2_lambda(const CT& C) { return C.cy_log2_lambda; }
+    template <class CT>
+    __device__ static double log2_tau_max(const CT& C) { return C.cy_log2_tau_max; }
+    template <class CT>
+    __device__ static double log2_tau_min(const CT& C) { return C.cy_log2_tau_min; }
 };
 template <>
 struct CyK<float> {
-    __device__ static float nu_ref(const CaseDev& C) { return C.nu_ref_f; }
-    __device__ static float nu0(const CaseDev& C) { return C.cy_nu0_f; }
-    __device__ static float a(const CaseDev& C) { return C.cy_a_f; }
-    __device__ static float n(const CaseDev& C) { return C.cy_n_f; }
-    __device__ static float e(const CaseDev& C) { return C.cy_e_f; }
-    __device__ static float log2_lambda(const CaseDev& C) { return C.cy_log2_lambda_f; }
-    __device__ static float log2_tau_max(const CaseDev& C) { return C.cy_log2_tau_max_f; }
-    __device__ static float log2_tau_min(const CaseDev& C) { return C.cy_log2_tau_min_f; }
+    template <class CT>
+    __device__ static float nu_ref(const CT& C) { return C.nu_ref_f; }
+    template <class CT>
+    __device__ static float nu0(const CT& C) { return C.cy_nu0_f; }
+    template <class CT>
+    __device__ static float a(const CT& C) { return C.cy_a_f; }
+    template <class CT>
+    __device__ static float n(const CT& C) { return C.cy_n_f; }
+    template <class CT>
+    __device__ static float e(const CT& C) { return C.cy_e_f; }
+    template <class CT>
+    __device__ static float log2_lambda(const CT& C) { return C.cy_log2_lambda_f; }
+    template <class CT>
+    __device__ static float log2_tau_max(const CT& C) { return C.cy_log2_tau_max_f; }
+    template <class CT>
+    __device__ static float log2_tau_min(const CT& C) { return C.cy_log2_tau_min_f; }
 };
 
 // local viscosity of the "neq" coupling: gamma from the deviatoric
@@ -630,8 +646,8 @@ struct CyK<float> {
 // Solved for t = log2(gamma) by Newton steps on H(t) = t + log2(tau) - log2(G)
 // from the middle of the bracket (gamma*tau is close to a power law, so H is
 // nearly linear in t); the oracle runs the same iteration (3 steps).
-template <class R>
-__device__ __forceinline__ R neq_viscosity(const Moments<R>& M, R ux, R uy, R uz, const CaseDev& C)
+template <class R, class CT>
+__device__ __forceinline__ R neq_viscosity(const Moments<R>& M, R ux, R uy, R uz, const CT& C)
 {
     const R rho = M.rho;
     const R Pxx = M.pxx - rho * ux * ux, Pyy = M.pyy - rho * uy * uy, Pzz = M.pzz - rho * uz * uz;
@@ -686,16 +702,16 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) { return (uint64_t(1) 
 
 // ---------------------------------------------------------------------------
 // the fused step
-template <int S, int MODEL, bool NT>
-__device__ __forceinline__ void step_brick(const StepLaunch& L, const BrickRef br, int lane)
+// one time step of brick b of case C at launch index `launch` (the case's
+// step index is launch - C.start)
+template <int S, int MODEL, bool NT, class CT>
+__device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, int lane)
 {
     constexpr int kEvenAux = NT ? kAuxNt : 0;
     using P = Pop<S>;
     using R = typename P::R;
-    const CaseDev& C = L.cases[br.case_id];
-    const int b = int(br.brick);
     const rsrc_t F = make_rsrc(C.f, C.f_bytes);
-    const int64_t step = *L.base + L.rel - C.start;
+    const int64_t step = launch - C.start;
     const int parity = int(step & 1);
     // per-step output decisions, wave-uniform (32-bit step arithmetic)
     const uint32_t next = uint32_t(step) + 1u;
@@ -952,12 +968,83 @@ template <int S, int MODEL, bool NT>
 __global__ void __launch_bounds__(512) k_step(StepLaunch L)
 {
     const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
-    const bool backwards = ((*L.base + L.rel) & 1) != 0;
+    const int64_t launch = *L.base + L.rel;
+    const bool backwards = (launch & 1) != 0;
     const int blk = xcd_block(int(blockIdx.x), int(gridDim.x), backwards);
     const int w = blk * L.bricks_per_group + wid;
     if (w < L.n_list) {
         const BrickRef br = L.list[w];
-        step_brick<S, MODEL, NT>(L, br, int(threadIdx.x) & 63);
+        step_brick<S, MODEL, NT>(L.cases[br.case_id], launch, int(br.brick), int(threadIdx.x) & 63);
+    }
+}
+
+// the batch descriptor, the brick list and the member constants are not
+// written while a step runs, so the batch kernels read them through the
+// constant address space (scalar loads into SGPRs)
+template <class T>
+using const_as = const __attribute__((address_space(4))) T;
+
+// Batches step through a device-resident descriptor (brick list, member
+// constants, the brick range of the group being stepped), so that a captured
+// graph of these launches stays valid when the membership changes between
+// replays, and one graph serves every group of the batch (the range is set
+// before each replay).  k_step_range keeps one brick per wavefront: the grid
+// (n_list workgroups, fixed at capture, at least the largest group) is split
+// into 8 XCD chunks over the range's ceil(n / bpg) workgroups as k_step does,
+// and the workgroups past the range exit at once.
+template <int S, int MODEL, bool NT>
+__global__ void __launch_bounds__(512) k_step_range(StepLaunch L)
+{
+    const_as<BatchDev>& B = *(const_as<BatchDev>*)L.batch;
+    const int n = B.end - B.begin;
+    const int64_t launch = *L.base + L.rel;
+    const bool backwards = (launch & 1) != 0;
+    const int bpg = L.bricks_per_group;
+    const int chunk = ((n + bpg - 1) / bpg + 7) >> 3;
+    const int pos = int(blockIdx.x) >> 3;
+    if (pos >= chunk) return;
+    const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const int w = ((int(blockIdx.x) & 7) * chunk + (backwards ? chunk - 1 - pos : pos)) * bpg + wid;
+    if (w < n) {
+        const_as<BrickRef>& br = ((const_as<BrickRef>*)B.list)[B.begin + w];
+        step_brick<S, MODEL, NT>(((const_as<CaseDev>*)B.cases)[br.case_id], launch, int(br.brick),
+                                 int(threadIdx.x) & 63);
+    }
+}
+
+// The persistent form for batches: a grid of a fixed size, multiple of 8,
+// whose wavefronts stride over the brick range [begin, end) of the batch
+// descriptor.  The brick list, the member constants and the range are read
+// from device memory, so a captured graph of these launches stays valid when
+// the batch's membership changes between replays, and one graph serves every
+// group of the batch (the range is set before each replay).  XCD x (blockIdx
+// % 8) walks the x-th eighth of the range, backwards on odd launches, as
+// k_step does; every wavefront leaves the loop when its next index passes
+// the end of its eighth.
+template <int S, int MODEL, bool NT>
+__global__ void __launch_bounds__(512) k_step_loop(StepLaunch L)
+{
+    const_as<BatchDev>& B = *(const_as<BatchDev>*)L.batch;
+    const_as<BrickRef>* const list = (const_as<BrickRef>*)B.list;
+    const_as<CaseDev>* const cases = (const_as<CaseDev>*)B.cases;
+    const int begin = B.begin, n = B.end - B.begin;
+    const int64_t launch = *L.base + L.rel;
+    const bool backwards = (launch & 1) != 0;
+    const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+    const int xcd = int(blockIdx.x) & 7;
+    const int chunk = (n + 7) >> 3;
+    const int c0 = xcd * chunk, c1 = min(c0 + chunk, n);
+    const int stride = (int(gridDim.x) >> 3) * L.bricks_per_group;
+#pragma nounroll
+    for (int k = (int(blockIdx.x) >> 3) * L.bricks_per_group + wid; c0 + k < c1; k += stride) {
+        const_as<BrickRef>& br = list[begin + (backwards ? c1 - 1 - k : c0 + k)];
+        // the lane's 54 neighbour offsets and edge masks depend on the lane
+        // only; hoisted out of the loop they would stay live across
+        // iterations (134 instead of 63 VGPRs, 3 instead of 8 wavefronts per
+        // SIMD), so the lane index is made opaque to loop-invariant motion
+        int lane = int(threadIdx.x) & 63;
+        asm volatile("" : "+v"(lane));
+        step_brick<S, MODEL, NT>(cases[br.case_id], launch, int(br.brick), lane);
     }
 }
 
@@ -1058,6 +1145,7 @@ __global__ void k_zero(uint32_t* p, int64_t n)
 // single-value device updates (kernel arguments are copied at launch, so the
 // host value can change right after the call; no memset/memcpy nodes)
 __global__ void k_set_u64(uint64_t* p, uint64_t v) { *p = v; }
+__global__ void k_set_batch(BatchDev* p, BatchDev v) { *p = v; }
 
 // ---------------------------------------------------------------------------
 // strain S of the velocity field with the reference stencil
@@ -1132,8 +1220,13 @@ __global__ void __launch_bounds__(64) k_strain(const CaseDev* Cp)
 // SVC/x+ plane, label-5 and label-6 planes (compute_flows.py:107-120, with
 // inlets reported as positive inflow, compute_flows.py:133-137); wavefront
 // shuffle reduction then one FP64 atomic per wavefront and plane.
+//   mean = 0: u of the BC cell from the macro output of the last step;
+//   mean = 1: the density-weighted mean momentum sum rho u (the mean_velocity
+//             output, sim_tcpc.cu:724-727) from the FP64 statistics sums plus
+//             the first `open` FP32 samples of the statistics ring (the open
+//             group); the caller divides by the sample count.
 template <class R>
-__global__ void __launch_bounds__(256) k_flux(const CaseDev* Cp, double* out)
+__global__ void __launch_bounds__(256) k_flux(const CaseDev* Cp, double* out, int mean, int open)
 {
     const CaseDev& C = *Cp;
     const int s = int(blockIdx.x * blockDim.x + threadIdx.x);
@@ -1150,10 +1243,18 @@ __global__ void __launch_bounds__(256) k_flux(const CaseDev* Cp, double* out)
         case L_OUT_YM: plane = 2; comp = 2; sign = -1.0; break; // outward = -y
         default: plane = 3; comp = 2; sign = 1.0; break;        // outward = +y
         }
-        // velocity of the BC cell from the macro output of the last step
         const int cell = C.bc_cell[s];
         const int b = cell >> 6, l = cell & 63;
-        const double u = double(static_cast<const R*>(C.macro)[(b * 4 + comp) * kLanes + l]);
+        double u;
+        if (!mean) {
+            u = double(static_cast<const R*>(C.macro)[(b * kMacroFields + comp) * kLanes + l]);
+        } else {
+            // rho u_comp: field comp - 1 of the sums and of each ring slot
+            u = C.stats[(int64_t(b) * kStatFields + comp - 1) * kLanes + l];
+            const int64_t slot = int64_t(C.n_active) * kRingFields * kLanes;
+            for (int k = 0; k < open; ++k)
+                u += double(C.stats_part[k * slot + (int64_t(b) * kRingFields + comp - 1) * kLanes + l]);
+        }
         v[plane] = u * sign;
     }
 #pragma unroll
@@ -1182,10 +1283,22 @@ template <int S, int MODEL>
 static int launch_step_t(const StepLaunch& L, hipStream_t st)
 {
     const int bpg = L.bricks_per_group;
+    const unsigned lds = unsigned(L.lds_bytes);
+    if (L.batch) {
+        // batch forms: L.n_list is the grid size in workgroups
+        const int groups = std::max(8, (L.n_list + 7) / 8 * 8);
+        if (L.per_wave > 1) {
+            if (L.nontemporal) hipLaunchKernelGGL((k_step_loop<S, MODEL, true>), dim3(groups), dim3(64 * bpg), lds, st, L);
+            else hipLaunchKernelGGL((k_step_loop<S, MODEL, false>), dim3(groups), dim3(64 * bpg), lds, st, L);
+        } else {
+            if (L.nontemporal) hipLaunchKernelGGL((k_step_range<S, MODEL, true>), dim3(groups), dim3(64 * bpg), lds, st, L);
+            else hipLaunchKernelGGL((k_step_range<S, MODEL, false>), dim3(groups), dim3(64 * bpg), lds, st, L);
+        }
+        return check_launch();
+    }
     int groups = (L.n_list + bpg - 1) / bpg;
     groups = ((groups + 7) / 8) * 8;
     if (groups == 0) groups = 8;
-    const unsigned lds = unsigned(L.lds_bytes);
     if (L.nontemporal) hipLaunchKernelGGL((k_step<S, MODEL, true>), dim3(groups), dim3(64 * bpg), lds, st, L);
     else hipLaunchKernelGGL((k_step<S, MODEL, false>), dim3(groups), dim3(64 * bpg), lds, st, L);
     return check_launch();
@@ -1239,13 +1352,13 @@ int launch_strain(int storage, const CaseDev* dev_case, int n_active, void* stre
     return check_launch();
 }
 
-int launch_flux(int storage, const CaseDev* dev_case, int n_bc, double* dev_out, void* stream)
+int launch_flux(int storage, const CaseDev* dev_case, int n_bc, double* dev_out, int mean, int open, void* stream)
 {
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (n_bc <= 0) return 0;
     const int nb = (n_bc + 255) / 256;
-    if (storage == S_F64) hipLaunchKernelGGL(k_flux<double>, dim3(nb), dim3(256), 0, st, dev_case, dev_out);
-    else hipLaunchKernelGGL(k_flux<float>, dim3(nb), dim3(256), 0, st, dev_case, dev_out);
+    if (storage == S_F64) hipLaunchKernelGGL(k_flux<double>, dim3(nb), dim3(256), 0, st, dev_case, dev_out, mean, open);
+    else hipLaunchKernelGGL(k_flux<float>, dim3(nb), dim3(256), 0, st, dev_case, dev_out, mean, open);
     return check_launch();
 }
 
@@ -1255,6 +1368,12 @@ int launch_zero(void* ptr, int64_t bytes, void* stream)
     const int64_t n = bytes / 4;
     if (n <= 0) return 0;
     hipLaunchKernelGGL(k_zero, dim3(int((n + 255) / 256)), dim3(256), 0, st, static_cast<uint32_t*>(ptr), n);
+    return check_launch();
+}
+
+int launch_set_batch(BatchDev* dev_ptr, const BatchDev& value, void* stream)
+{
+    hipLaunchKernelGGL(k_set_batch, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), dev_ptr, value);
     return check_launch();
 }
 

@@ -2,13 +2,17 @@
 // against a device-memory budget and stepped together.
 //
 // The reference runs one case per process and queues cases through Slurm
-// (scripts/slurm_run_cases.py:104-198, scripts/run_all_cases.py).  Here one
-// worker thread owns the device: requests wait in FIFO order until their
-// footprint fits the budget, admitted cases of one collision/storage type are
-// stepped by one fused launch per step over their concatenated brick lists
-// (a batch), batches of different types are queued on their own streams in
-// the same tick, and finished cases leave between ticks so that waiting ones
-// can take their memory (continuous batching).
+// (scripts/slurm_run_cases.py:104-198, scripts/run_all_cases.py).  Here a
+// worker thread drives the device: requests wait in FIFO order until their
+// footprint fits the budget; a builder thread creates admitted cases (host
+// setup and uploads) and frees retired ones, so the running cases keep
+// stepping meanwhile; created cases join the long-lived batch of their
+// collision/storage type at a tick boundary (one fused launch per step over
+// the concatenated brick lists, membership updated in device memory without
+// re-capturing the batch's graph); batches of different types are queued on
+// their own streams in the same tick; finished cases leave their batch at the
+// end of their last tick, so that waiting ones can take their memory
+// (continuous batching).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +37,7 @@ int batch_enqueue(lbm355_batch* b, int64_t n);
 int batch_finish(lbm355_batch* b);
 int lbm355_set_error(int code, const std::string& msg);
 int64_t lbm355_case_step_bytes(const lbm355_case* c);
+int batch_update(lbm355_batch* b, const std::vector<lbm355_case*>& add, const std::vector<lbm355_case*>& remove);
 
 namespace {
 
@@ -50,6 +55,8 @@ struct Request {
     lbm355_request_status st{};
     bool admitted = false;  // its footprint is counted in the pool's used bytes
     bool taken = false;
+    int build_rc = LBM355_OK;   // result of the case creation (builder thread)
+    std::string message;        // error message of a failed creation or tick
 };
 
 double now_s(std::chrono::steady_clock::time_point t0)
@@ -61,8 +68,8 @@ double now_s(std::chrono::steady_clock::time_point t0)
 
 struct lbm355_pool {
     std::mutex mu;
-    std::condition_variable wake, changed;
-    std::thread worker;
+    std::condition_variable wake, changed, build_cv;
+    std::thread worker, builder;
     bool stop = false;
     int device = 0;
     int64_t budget = 0, used = 0;
@@ -70,24 +77,30 @@ struct lbm355_pool {
     int graph = 1;
     std::chrono::steady_clock::time_point t0;
     std::map<int64_t, std::unique_ptr<Request>> reqs;
-    std::deque<int64_t> queue;
-    std::vector<Request*> running;  // worker-owned
+    std::deque<int64_t> queue;           // submitted, waiting for admission
+    std::deque<Request*> to_build;       // admitted, waiting for case creation (builder thread)
+    std::deque<Request*> built;          // created or failed, waiting for the worker
+    std::deque<lbm355_case*> to_destroy; // retired cases (the builder frees them)
+    std::vector<Request*> running;       // worker-owned
     int64_t next_id = 1;
-    int64_t n_done = 0, n_failed = 0, n_running = 0;
+    int64_t n_done = 0, n_failed = 0, n_running = 0, n_building = 0;
     int64_t cell_updates = 0, ticks = 0;
     double busy = 0.0;
     std::string worker_error;
 
     void run();
+    void build_loop();
     void finish(Request* r, int state, int error);
 };
 
 void lbm355_pool::finish(Request* r, int state, int error)
 {
-    // called with the lock held; the worker's last library error explains a failure
+    // called with the lock held; a failed request carries the library's
+    // error message of the failing call
     if (state == LBM355_REQ_FAILED) {
         const char* m = error == LBM355_ENOMEM && r->st.t_start == 0.0 ? "footprint exceeds the pool budget"
-                                                                        : lbm355_last_error();
+                        : r->message.empty()                           ? lbm355_last_error()
+                                                                       : r->message.c_str();
         std::strncpy(r->st.message, m, sizeof r->st.message - 1);
     }
     r->st.state = state;
@@ -102,24 +115,99 @@ void lbm355_pool::finish(Request* r, int state, int error)
     changed.notify_all();
 }
 
+// case creation (host labels, brick index, host tables and device uploads,
+// about 0.3 s at 512x384x256) and case destruction run here, so that the
+// worker that drives the GPU keeps stepping the running cases meanwhile
+void lbm355_pool::build_loop()
+{
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        build_cv.wait(lk, [&] { return stop || !to_build.empty() || !to_destroy.empty(); });
+        if (!to_destroy.empty()) {
+            lbm355_case* c = to_destroy.front();
+            to_destroy.pop_front();
+            lk.unlock();
+            lbm355_case_destroy(c);
+            lk.lock();
+            continue;
+        }
+        if (stop) break;
+        Request* r = to_build.front();
+        to_build.pop_front();
+        lk.unlock();
+        r->desc.geometry = r->geometry.data();
+        r->build_rc = lbm355_case_create(&r->desc, &r->c);
+        if (r->build_rc != LBM355_OK) {
+            r->message = lbm355_last_error();
+            if (r->c) lbm355_case_destroy(r->c);  // zero-area cases terminate like sim_tcpc (exit code 2)
+            r->c = nullptr;
+        } else {
+            lbm355_case_info info;
+            std::memset(&info, 0, sizeof info);
+            lbm355_case_info_get(r->c, &info);
+            r->active_cells = info.active_cells;
+            r->st.active_cells = info.active_cells;
+            r->st.device_bytes = info.device_bytes;
+        }
+        lk.lock();
+        built.push_back(r);
+        wake.notify_all();
+    }
+}
+
 void lbm355_pool::run()
 {
     (void)hipSetDevice(device);
-    std::vector<lbm355_batch*> batches;
-    std::vector<lbm355_case*> solo;  // fd-force cases: their pre-kernels are per case
-    bool rebuild = false;
-    auto drop_batches = [&]() {
-        for (lbm355_batch* b : batches) lbm355_batch_destroy(b);
-        batches.clear();
-        solo.clear();
+    // one long-lived batch per collision/storage type; fd-force cases step on
+    // their own streams (their pre-kernels are per case)
+    std::map<std::pair<int, int>, lbm355_batch*> batches;
+    std::vector<Request*> solo;
+    auto key = [](const Request* r) { return std::make_pair(r->desc.storage, r->desc.collision); };
+    auto fd_force = [](const Request* r) { return r->desc.cy_enabled && r->desc.cy_model == LBM355_CY_FD_FORCE; };
+    // membership changes of every batch with one upload each (a batch that
+    // becomes empty is destroyed); the first error is returned
+    auto apply = [&](const std::vector<Request*>& add, const std::vector<Request*>& remove) {
+        std::map<std::pair<int, int>, std::pair<std::vector<lbm355_case*>, std::vector<lbm355_case*>>> by;
+        for (Request* r : remove) {
+            if (fd_force(r)) {
+                auto it = std::find(solo.begin(), solo.end(), r);
+                if (it != solo.end()) solo.erase(it);
+            } else {
+                by[key(r)].second.push_back(r->c);
+            }
+        }
+        for (Request* r : add) {
+            if (fd_force(r)) solo.push_back(r);
+            else by[key(r)].first.push_back(r->c);
+        }
+        int rc = LBM355_OK;
+        for (auto& kv : by) {
+            auto it = batches.find(kv.first);
+            if (it == batches.end()) {
+                if (kv.second.first.empty()) continue;
+                lbm355_batch* b = nullptr;
+                rc = lbm355_batch_create(kv.second.first.data(), int(kv.second.first.size()), graph, &b);
+                if (rc != LBM355_OK) break;
+                batches[kv.first] = b;
+            } else {
+                if ((rc = batch_update(it->second, kv.second.first, kv.second.second)) != LBM355_OK) break;
+                int members = 0;
+                lbm355_batch_size(it->second, &members, nullptr);
+                if (members == 0) {
+                    lbm355_batch_destroy(it->second);
+                    batches.erase(it);
+                }
+            }
+        }
+        return rc;
     };
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-        wake.wait(lk, [&] { return stop || !queue.empty() || !running.empty(); });
+        wake.wait(lk, [&] { return stop || !queue.empty() || !running.empty() || !built.empty(); });
         if (stop) break;
         // FIFO admission against the budget; a request larger than the whole
         // budget fails instead of blocking the queue
-        std::vector<Request*> admit;
         while (!queue.empty()) {
             Request* r = reqs[queue.front()].get();
             if (r->footprint > budget) {
@@ -133,151 +221,134 @@ void lbm355_pool::run()
             r->admitted = true;
             r->st.state = LBM355_REQ_RUNNING;
             r->st.t_start = now_s(t0);
-            admit.push_back(r);
+            to_build.push_back(r);
+            ++n_building;
+            build_cv.notify_all();
         }
+        // the cases the builder finished join their batches at this tick boundary
+        std::vector<Request*> fresh(built.begin(), built.end());
+        built.clear();
+        n_building -= int64_t(fresh.size());
         lk.unlock();
 
-        for (Request* r : admit) {
-            r->desc.geometry = r->geometry.data();
-            const int rc = lbm355_case_create(&r->desc, &r->c);
-            lbm355_case_info info;
-            std::memset(&info, 0, sizeof info);
-            if (rc == LBM355_OK) lbm355_case_info_get(r->c, &info);
-            if (rc != LBM355_OK) {
-                // zero-area cases terminate like sim_tcpc (exit code 2)
-                if (r->c) lbm355_case_destroy(r->c);
-                r->c = nullptr;
-                lk.lock();
-                finish(r, LBM355_REQ_FAILED, rc);
-                lk.unlock();
-                continue;
-            }
-            r->active_cells = info.active_cells;
-            r->st.active_cells = info.active_cells;
-            r->st.device_bytes = info.device_bytes;
-            running.push_back(r);
-            rebuild = true;
-        }
-        if (running.empty()) {
-            lk.lock();
-            n_running = 0;
-            continue;
-        }
-        if (rebuild) {
-            drop_batches();
-            std::map<std::pair<int, int>, std::vector<lbm355_case*>> groups;
-            for (Request* r : running) {
-                if (r->desc.cy_enabled && r->desc.cy_model == LBM355_CY_FD_FORCE) solo.push_back(r->c);
-                else groups[{r->desc.storage, r->desc.collision}].push_back(r->c);
-            }
-            int rc = LBM355_OK;
-            for (auto& g : groups) {
-                lbm355_batch* b = nullptr;
-                rc = lbm355_batch_create(g.second.data(), int(g.second.size()), graph, &b);
-                if (rc != LBM355_OK) break;
-                batches.push_back(b);
-            }
-            if (rc != LBM355_OK) {
-                lk.lock();
-                worker_error = lbm355_last_error();
-                for (Request* r : running) {
-                    lbm355_case_destroy(r->c);
-                    r->c = nullptr;
-                    finish(r, LBM355_REQ_FAILED, rc);
-                }
-                running.clear();
-                n_running = 0;
-                drop_batches();
-                continue;
-            }
-            rebuild = false;
-        }
-        // one tick: the same step count for every running case, cut at the
-        // next completion or report point
-        int64_t n = tick;
-        for (Request* r : running) {
-            n = std::min(n, r->steps - r->st.steps_done);
-            if (r->report_every > 0) n = std::min(n, r->report_every - r->st.steps_done % r->report_every);
-        }
-        n = std::max<int64_t>(n, 1);
-        const auto t_tick = std::chrono::steady_clock::now();
-        int rc = LBM355_OK;
-        // while all running cases fit the Infinity Cache together, the
-        // batches of a tick run concurrently on their streams; beyond that
-        // they run one after another, so that each batch's cache-sized groups
-        // do not evict each other
-        int64_t resident = 0;
-        for (Request* r : running) resident += lbm355_case_step_bytes(r->c);
-        const bool serial = resident > lbm355::kBatchGroupBytes;
-        for (lbm355_batch* b : batches) {
-            if (rc == LBM355_OK) rc = batch_enqueue(b, n);
-            if (serial) {
-                const int r2 = batch_finish(b);
-                if (rc == LBM355_OK) rc = r2;
-            }
-        }
-        for (lbm355_case* c : solo) {
-            if (rc == LBM355_OK) rc = lbm355_case_step(c, n);  // queued on the case's stream
-            if (serial) {
-                const int r2 = lbm355_case_sync(c);
-                if (rc == LBM355_OK) rc = r2;
-            }
-        }
-        for (lbm355_batch* b : batches) {
-            const int r2 = batch_finish(b);
-            if (rc == LBM355_OK) rc = r2;
-        }
-        for (lbm355_case* c : solo) {
-            const int r2 = lbm355_case_sync(c);
-            if (rc == LBM355_OK) rc = r2;
-        }
-        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_tick).count();
-        std::vector<Request*> still;
+        std::vector<Request*> joined;
         std::vector<std::pair<Request*, int>> ended;
-        int64_t updates = 0;
-        for (Request* r : running) {
-            if (rc != LBM355_OK) { ended.push_back({r, rc}); continue; }
-            r->st.steps_done += n;
-            updates += r->active_cells * n;
-            const bool report = r->report_every > 0 && r->st.steps_done % r->report_every == 0;
-            const bool done = r->st.steps_done >= r->steps;
-            if (report || done) {
-                double fl[4] = {0, 0, 0, 0};
-                if (lbm355_case_flux(r->c, 0, fl) == LBM355_OK) {
-                    std::memcpy(r->st.flux, fl, sizeof fl);
-                    if (r->st.t_first_report == 0.0) r->st.t_first_report = now_s(t0);
+        for (Request* r : fresh) {
+            if (r->build_rc != LBM355_OK) ended.push_back({r, r->build_rc});
+            else joined.push_back(r);
+        }
+        int rc = apply(joined, {});
+        if (rc != LBM355_OK) {
+            worker_error = lbm355_last_error();
+            for (Request* r : joined) {
+                r->message = worker_error;
+                ended.push_back({r, rc});
+            }
+            apply({}, joined);  // whatever did join leaves again
+        } else {
+            running.insert(running.end(), joined.begin(), joined.end());
+        }
+        int64_t n = 0, updates = 0;
+        double dt = 0.0;
+        if (!running.empty()) {
+            // one tick: the same step count for every running case, cut at the
+            // next completion or report point
+            n = tick;
+            for (Request* r : running) {
+                n = std::min(n, r->steps - r->st.steps_done);
+                if (r->report_every > 0) n = std::min(n, r->report_every - r->st.steps_done % r->report_every);
+            }
+            n = std::max<int64_t>(n, 1);
+            const auto t_tick = std::chrono::steady_clock::now();
+            rc = LBM355_OK;
+            // while all running cases fit the Infinity Cache together, the
+            // batches of a tick run concurrently on their streams; beyond that
+            // they run one after another, so that each batch's cache-sized
+            // groups do not evict each other
+            int64_t resident = 0;
+            for (Request* r : running) resident += lbm355_case_step_bytes(r->c);
+            const bool serial = resident > lbm355::kBatchGroupBytes;
+            for (auto& kv : batches) {
+                if (rc == LBM355_OK) rc = batch_enqueue(kv.second, n);
+                if (serial) {
+                    const int r2 = batch_finish(kv.second);
+                    if (rc == LBM355_OK) rc = r2;
                 }
             }
-            if (done) {
-                double mf[4] = {0, 0, 0, 0};
-                if (r->desc.stats_every > 0 && lbm355_case_flux(r->c, 1, mf) == LBM355_OK)
-                    std::memcpy(r->st.mean_flux, mf, sizeof mf);
-                ended.push_back({r, LBM355_OK});
-            } else {
-                still.push_back(r);
+            for (Request* r : solo) {
+                if (rc == LBM355_OK) rc = lbm355_case_step(r->c, n);  // queued on the case's stream
+                if (serial) {
+                    const int r2 = lbm355_case_sync(r->c);
+                    if (rc == LBM355_OK) rc = r2;
+                }
             }
-        }
-        for (auto& e : ended)
-            if (!(e.first->keep && e.second == LBM355_OK)) {
-                lbm355_case_destroy(e.first->c);
-                e.first->c = nullptr;
+            for (auto& kv : batches) {
+                const int r2 = batch_finish(kv.second);
+                if (rc == LBM355_OK) rc = r2;
             }
-        if (!ended.empty()) {
-            rebuild = true;
-            drop_batches();  // a batch must not outlive one of its cases
+            for (Request* r : solo) {
+                const int r2 = lbm355_case_sync(r->c);
+                if (rc == LBM355_OK) rc = r2;
+            }
+            dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_tick).count();
+            if (rc != LBM355_OK) worker_error = lbm355_last_error();
+            std::vector<Request*> still, leaving;
+            for (Request* r : running) {
+                if (rc != LBM355_OK) {
+                    r->message = worker_error;
+                    ended.push_back({r, rc});
+                    leaving.push_back(r);
+                    continue;
+                }
+                r->st.steps_done += n;
+                updates += r->active_cells * n;
+                const bool report = r->report_every > 0 && r->st.steps_done % r->report_every == 0;
+                const bool done = r->st.steps_done >= r->steps;
+                if (report || done) {
+                    double fl[4] = {0, 0, 0, 0};
+                    if (lbm355_case_flux(r->c, 0, fl) == LBM355_OK) {
+                        std::memcpy(r->st.flux, fl, sizeof fl);
+                        if (r->st.t_first_report == 0.0) r->st.t_first_report = now_s(t0);
+                    }
+                }
+                if (done) {
+                    double mf[4] = {0, 0, 0, 0};
+                    if (r->desc.stats_every > 0 && lbm355_case_flux(r->c, 1, mf) == LBM355_OK)
+                        std::memcpy(r->st.mean_flux, mf, sizeof mf);
+                    ended.push_back({r, LBM355_OK});
+                    leaving.push_back(r);
+                } else {
+                    still.push_back(r);
+                }
+            }
+            // finished cases leave their batches before anyone can take them
+            if (apply({}, leaving) != LBM355_OK && worker_error.empty()) worker_error = lbm355_last_error();
+            running.swap(still);
         }
-        running.swap(still);
         lk.lock();
-        ticks += 1;
-        busy += dt;
-        cell_updates += updates;
+        for (auto& e : ended) {
+            Request* r = e.first;
+            if (r->c && !(r->keep && e.second == LBM355_OK)) {
+                to_destroy.push_back(r->c);
+                r->c = nullptr;
+                build_cv.notify_all();
+            }
+            finish(r, e.second == LBM355_OK ? LBM355_REQ_DONE : LBM355_REQ_FAILED, e.second);
+        }
+        if (n > 0) {
+            ticks += 1;
+            busy += dt;
+            cell_updates += updates;
+        }
         n_running = int64_t(running.size());
-        for (auto& e : ended) finish(e.first, e.second == LBM355_OK ? LBM355_REQ_DONE : LBM355_REQ_FAILED, e.second);
         changed.notify_all();
     }
     lk.unlock();
-    drop_batches();
-    for (Request* r : running) lbm355_case_destroy(r->c);
+    for (auto& kv : batches) lbm355_batch_destroy(kv.second);
+    for (Request* r : running) {
+        lbm355_case_destroy(r->c);
+        r->c = nullptr;
+    }
     running.clear();
 }
 
@@ -304,6 +375,7 @@ int lbm355_pool_create(int64_t budget_bytes, int steps_per_tick, int graph, lbm3
     p->t0 = std::chrono::steady_clock::now();
     lbm355_pool* raw = p.get();
     p->worker = std::thread([raw] { raw->run(); });
+    p->builder = std::thread([raw] { raw->build_loop(); });
     *out = p.release();
     return LBM355_OK;
 }
@@ -369,7 +441,9 @@ int lbm355_pool_drain(lbm355_pool* p, double timeout_s)
 {
     if (!p) return pool_fail(LBM355_EINVAL, "null pool");
     std::unique_lock<std::mutex> lk(p->mu);
-    auto idle = [&] { return p->queue.empty() && p->n_running == 0 && p->n_done + p->n_failed == p->next_id - 1; };
+    auto idle = [&] {
+        return p->queue.empty() && p->n_running == 0 && p->n_building == 0 && p->n_done + p->n_failed == p->next_id - 1;
+    };
     if (timeout_s < 0) p->changed.wait(lk, idle);
     else p->changed.wait_for(lk, std::chrono::duration<double>(timeout_s), idle);
     return idle() ? LBM355_OK : LBM355_EBUSY;
@@ -421,6 +495,18 @@ int lbm355_pool_destroy(lbm355_pool* p)
         p->wake.notify_all();
     }
     if (p->worker.joinable()) p->worker.join();
+    {
+        std::lock_guard<std::mutex> g(p->mu);
+        p->build_cv.notify_all();
+    }
+    if (p->builder.joinable()) p->builder.join();
+    // cases built after the worker stopped, and retired ones not freed yet
+    for (Request* r : p->built)
+        if (r->c) {
+            lbm355_case_destroy(r->c);
+            r->c = nullptr;
+        }
+    for (lbm355_case* c : p->to_destroy) lbm355_case_destroy(c);
     for (auto& kv : p->reqs)
         if (kv.second->c) lbm355_case_destroy(kv.second->c);  // kept and never taken
     delete p;

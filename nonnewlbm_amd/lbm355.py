@@ -191,9 +191,36 @@ class Batch:
         """Step members in groups of at most nbytes of device memory (0: one group)."""
         check(self.lib.lbm355_batch_set_group_bytes(self.h, int(nbytes)), "batch_set_group_bytes")
 
+    def add(self, case: Case) -> None:
+        """Make ``case`` a member (between batch calls); it keeps its own step count."""
+        check(self.lib.lbm355_batch_add(self.h, case.h), "batch_add")
+        self.cases.append(case)
+
+    def remove(self, case: Case) -> None:
+        check(self.lib.lbm355_batch_remove(self.h, case.h), "batch_remove")
+        self.cases = [c for c in self.cases if c is not case]
+
+    def size(self) -> tuple[int, int]:
+        """(members, bricks in the concatenated list)."""
+        m, b = C.c_int(), C.c_int64()
+        check(self.lib.lbm355_batch_size(self.h, C.byref(m), C.byref(b)), "batch_size")
+        return m.value, b.value
+
+    def set_per_wave(self, per_wave: int) -> None:
+        """Bricks per wavefront: 1 one each, k > 1 the persistent loop."""
+        check(self.lib.lbm355_batch_set_per_wave(self.h, int(per_wave)), "batch_set_per_wave")
+
+    def config(self) -> dict:
+        out = (C.c_int * 8)()
+        check(self.lib.lbm355_batch_config(self.h, out), "batch_config")
+        keys = ("bricks_per_group", "nontemporal", "waves_per_cu", "group_mib", "per_wave", "groups", "uploads",
+                "captures")
+        return dict(zip(keys, list(out)))
+
     def autotune(self) -> tuple[int, int, int, int]:
         """Returns the fastest (bricks_per_group, nontemporal, waves_per_cu,
-        group MiB with 0 = one group), now in use; 598 real steps with statistics every step."""
+        group MiB with 0 = one group), now in use (``config()`` also gives the
+        bricks per wavefront); real steps of every member."""
         best = C.c_int()
         check(self.lib.lbm355_batch_autotune(self.h, C.byref(best)), "batch_autotune")
         v = best.value

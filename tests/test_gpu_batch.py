@@ -143,3 +143,56 @@ def test_batch_waits_for_work_queued_on_a_case(gpu_lib):
     for k in range(3):
         for name in ("rho", "vx", "vy", "sum_rux", "sum_rruyy"):
             np.testing.assert_array_equal(mixed[k].field(name), alone[k].field(name), err_msg=f"case {k} {name}")
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_membership_changes(gpu_lib, graph):
+    """Members leave and join between batch calls (the device brick list and
+    member constants are re-uploaded; the captured graph is kept unless the
+    launch grid must grow): every case equals the same case stepped alone,
+    bit for bit, whatever it did in between."""
+    specs = _cases(_abi.F32)
+    alone = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
+    mixed = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
+    b = Batch(mixed[:2], graph=graph)
+    b.step(70)
+    b.remove(mixed[0])
+    assert b.size()[0] == 1
+    mixed[0].step(9)            # on its own meanwhile
+    b.add(mixed[2])             # a fresh case joins at its own step 0
+    b.step(66)
+    b.add(mixed[0])             # and the first one comes back
+    b.step(65)
+    cfg = b.config()
+    b.close()
+    steps = [70 + 9 + 65, 70 + 66 + 65, 66 + 65]
+    for c, n in zip(alone, steps):
+        c.step(n)
+    for k in range(3):
+        assert mixed[k].info()["steps"] == steps[k]
+        for name in ("rho", "vx", "vy", "vz"):
+            np.testing.assert_array_equal(mixed[k].field(name), alone[k].field(name), err_msg=f"case {k} {name}")
+    assert cfg["uploads"] == 4
+    if graph:
+        assert 1 <= cfg["captures"] <= 3
+
+
+@pytest.mark.parametrize("per_wave", [2, 4])
+def test_batch_persistent_loop_equals_alone(gpu_lib, per_wave):
+    """The persistent form (a grid of about 1/k of the bricks whose
+    wavefronts loop over the group) gives the bits of one wavefront per
+    brick."""
+    specs = _cases(_abi.F16_OFFSET)
+    alone = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
+    mixed = [Case(g.flat(), g.dims, **kw) for g, kw in specs]
+    b = Batch(mixed, graph=True)
+    b.set_per_wave(per_wave)
+    b.set_group_bytes(1)        # every member a group of its own
+    b.step(67)
+    assert b.config()["per_wave"] == per_wave and b.config()["groups"] == 3
+    b.close()
+    for c in alone:
+        c.step(67)
+    for k in range(3):
+        for name in ("rho", "vx", "vz"):
+            np.testing.assert_array_equal(mixed[k].field(name), alone[k].field(name), err_msg=f"case {k} {name}")
