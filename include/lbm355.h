@@ -384,6 +384,18 @@ int lbm355_pool_stats(lbm355_pool* p, lbm355_pool_info* st);
 /* stops the worker after its current tick; unfinished requests are dropped */
 int lbm355_pool_destroy(lbm355_pool* p);
 
+/* error of a case against a reference case made from the same geometry (for
+   example the same junction stored in FP16 against FP64), from the macro
+   output of the last step of each, reduced on the device:
+   out = {sum |u - u_ref|^2, sum |u_ref|^2, sum (rho - rho_ref)^2,
+          sum (rho_ref - 1)^2} over the non-wall cells, so that
+   sqrt(out[0] / out[1]) is the relative L2 error of velocity.  No counterpart
+   in the reference (the precision pattern states each field's error against
+   an FP64 run).  Allocates a partial-sum buffer of 4 ceil(active bricks / 4)
+   + 4 doubles for the call.  EINVAL when grid, labels or brick index differ
+   or either case has not stepped. */
+int lbm355_case_compare(lbm355_case* a, lbm355_case* ref, double out[4]);
+
 /* bytes of device memory a descriptor would need (admission control) */
 int lbm355_case_footprint(const lbm355_case_desc* desc, int64_t* bytes);
 /* the label setup of lbm355_case_create without a device: labels (nx*ny*nz,

@@ -1202,6 +1202,49 @@ int lbm355_case_flux(lbm355_case* c, int mean, double out[4])
     return LBM355_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// compare needs the same brick index on both sides: the same grid, labels,
+// origin shift and active bricks (cases made from one geometry and face table)
+int same_layout(const lbm355_case& a, const lbm355_case& b)
+{
+    const bool same = a.d.nx == b.d.nx && a.d.ny == b.d.ny && a.d.nz == b.d.nz && a.ox == b.ox && a.oy == b.oy &&
+                      a.n_active == b.n_active && a.lab.map == b.lab.map &&
+                      std::equal(a.brick_xyz.begin(), a.brick_xyz.begin() + 3 * int64_t(a.n_active), b.brick_xyz.begin());
+    return same ? LBM355_OK : fail(LBM355_EINVAL, "cases differ in grid, labels or brick index (compare needs one geometry)");
+}
+
+// the two passes of the difference norms on a's stream (after ref's queued
+// work), partials holding at least 4 * compare_blocks + 4 doubles
+int compare_into(lbm355_case* a, lbm355_case* ref, double* partials, double out[4])
+{
+    int rc;
+    if ((rc = same_layout(*a, *ref))) return rc;
+    if ((rc = refresh_macro(*a)) || (rc = refresh_macro(*ref))) return rc;
+    if (a->steps == 0 || ref->steps == 0) return fail(LBM355_EINVAL, "compare needs a step on both cases");
+    HIP_TRY(hipEventRecord(ref->joined, ref->stream));
+    HIP_TRY(hipStreamWaitEvent(a->stream, ref->joined, 0));
+    LAUNCH_TRY(launch_compare(a->storage, ref->storage, static_cast<const CaseDev*>(a->devcase.p),
+                              static_cast<const CaseDev*>(ref->devcase.p), a->n_active, partials, a->stream));
+    const int nb = compare_blocks(a->n_active);
+    HIP_TRY(hipMemcpyAsync(out, partials + 4 * nb, 4 * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+    HIP_TRY(hipStreamSynchronize(a->stream));
+    return LBM355_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int lbm355_case_compare(lbm355_case* a, lbm355_case* ref, double out[4])
+{
+    if (!a || !ref || !out) return fail(LBM355_EINVAL, "null argument");
+    DevBuf partials;
+    if (partials.alloc(size_t(4 * compare_blocks(a->n_active) + 4) * sizeof(double)) != 0)
+        return fail(LBM355_ENOMEM, "compare partials");
+    return compare_into(a, ref, static_cast<double*>(partials.p), out);
+}
+
 int lbm355_case_reset_stats(lbm355_case* c)
 {
     if (!c) return fail(LBM355_EINVAL, "null case");

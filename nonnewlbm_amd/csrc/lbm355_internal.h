@@ -186,6 +186,13 @@ int launch_fd_force_prepass(int storage, const StepLaunch& L, const CaseDev* dev
 // plane fluxes of the BC cells into dev_out[4] (accumulated: clear it first);
 // mean = 1 sums the statistics (FP64 sums + `open` ring samples) instead of u
 int launch_flux(int storage, const CaseDev* dev_case, int n_bc, double* dev_out, int mean, int open, void* stream);
+// difference norms of two cases with the same brick index into
+// partials[4 * compare_blocks(n_active) + 0..3] (the partials before them are
+// scratch of the same call): sum |u_a - u_b|^2, |u_b|^2, (rho_a - rho_b)^2,
+// (rho_b - 1)^2 over the non-wall cells
+int compare_blocks(int n_active);
+int launch_compare(int storage_a, int storage_b, const CaseDev* a, const CaseDev* b, int n_active, double* partials,
+                   void* stream);
 int launch_set_u64(void* dev_ptr, uint64_t value, void* stream);
 int launch_set_batch(BatchDev* dev_ptr, const BatchDev& value, void* stream);
 int launch_zero(void* ptr, int64_t bytes, void* stream);

@@ -1266,6 +1266,77 @@ __global__ void __launch_bounds__(256) k_flux(const CaseDev* Cp, double* out, in
 }
 
 // ---------------------------------------------------------------------------
+// difference norms of two cases with the same brick index (the same geometry
+// and grid): per non-wall cell |u_a - u_b|^2, |u_b|^2, (rho_a - rho_b)^2 and
+// (rho_b - 1)^2 from the macro output of their last steps.  Pass 1: each
+// workgroup of kCompareWaves wavefronts takes kCompareWaves consecutive
+// bricks and writes its 4 sums to partials[4 * blockIdx + k] (every
+// workgroup writes all 4, so nothing has to start at zero); pass 2: one
+// workgroup adds the partials in a fixed order into partials[4 * nblocks + k].
+// The summation order depends on the brick count only, so the result is the
+// same bits whatever buffer holds the partials.
+constexpr int kCompareWaves = 4;
+
+template <class Ra, class Rb>
+__global__ void __launch_bounds__(64 * kCompareWaves) k_compare(const CaseDev* A, const CaseDev* B, double* partials)
+{
+    __shared__ double red[kCompareWaves][4];
+    const int wid = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+    const int b = int(blockIdx.x) * kCompareWaves + wid;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (b < B->n_active) {
+        const int lab = B->label[b * kLanes + lane];
+        if (lab != L_WALL && lab != L_OUTSIDE) {
+            const Ra* ma = static_cast<const Ra*>(A->macro);
+            const Rb* mb = static_cast<const Rb*>(B->macro);
+            double d2 = 0.0, r2 = 0.0;
+            for (int c = 1; c <= 3; ++c) {
+                const double ua = double(ma[(b * kMacroFields + c) * kLanes + lane]);
+                const double ub = double(mb[(b * kMacroFields + c) * kLanes + lane]);
+                d2 += (ua - ub) * (ua - ub);
+                r2 += ub * ub;
+            }
+            const double ra = double(ma[b * kMacroFields * kLanes + lane]);
+            const double rb = double(mb[b * kMacroFields * kLanes + lane]);
+            v[0] = d2;
+            v[1] = r2;
+            v[2] = (ra - rb) * (ra - rb);
+            v[3] = (rb - 1.0) * (rb - 1.0);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) red[wid][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double x = 0.0;
+        for (int w = 0; w < kCompareWaves; ++w) x += red[w][threadIdx.x];
+        partials[4 * blockIdx.x + threadIdx.x] = x;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_sum_partials(double* partials, int nblocks)
+{
+    __shared__ double red[4][4];
+    const int lane = int(threadIdx.x) & 63, wid = int(threadIdx.x) >> 6;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = int(threadIdx.x); i < nblocks; i += int(blockDim.x))
+        for (int k = 0; k < 4; ++k) v[k] += partials[4 * i + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double x = v[k];
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+        if (lane == 0) red[wid][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) partials[4 * nblocks + threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] +
+                                                               red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------
 static thread_local const char* g_kerr = "";
 const char* kernel_error() { return g_kerr; }
 
@@ -1359,6 +1430,23 @@ int launch_flux(int storage, const CaseDev* dev_case, int n_bc, double* dev_out,
     const int nb = (n_bc + 255) / 256;
     if (storage == S_F64) hipLaunchKernelGGL(k_flux<double>, dim3(nb), dim3(256), 0, st, dev_case, dev_out, mean, open);
     else hipLaunchKernelGGL(k_flux<float>, dim3(nb), dim3(256), 0, st, dev_case, dev_out, mean, open);
+    return check_launch();
+}
+
+int compare_blocks(int n_active) { return std::max(1, (n_active + kCompareWaves - 1) / kCompareWaves); }
+
+int launch_compare(int storage_a, int storage_b, const CaseDev* a, const CaseDev* b, int n_active, double* partials,
+                   void* stream)
+{
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int nb = compare_blocks(n_active);
+    const bool da = storage_a == S_F64, db = storage_b == S_F64;
+    const dim3 grid(nb), block(64 * kCompareWaves);
+    if (da && db) hipLaunchKernelGGL((k_compare<double, double>), grid, block, 0, st, a, b, partials);
+    else if (da) hipLaunchKernelGGL((k_compare<double, float>), grid, block, 0, st, a, b, partials);
+    else if (db) hipLaunchKernelGGL((k_compare<float, double>), grid, block, 0, st, a, b, partials);
+    else hipLaunchKernelGGL((k_compare<float, float>), grid, block, 0, st, a, b, partials);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partials, nb);
     return check_launch();
 }
 

@@ -133,6 +133,16 @@ class Case:
         check(self.lib.lbm355_case_flux(self.h, 1 if mean else 0, out), "flux")
         return np.array(out[:], dtype=np.float64)
 
+    def compare(self, ref: "Case") -> dict:
+        """Relative L2 errors against ``ref`` (same geometry), reduced on the
+        device from both cases' last macro output: velocity, rho - 1, and the
+        four raw sums."""
+        out = (C.c_double * 4)()
+        check(self.lib.lbm355_case_compare(self.h, ref.h, out), "case_compare")
+        s = list(out)
+        return {"u_rel_l2": (s[0] / s[1]) ** 0.5 if s[1] > 0 else s[0] ** 0.5,
+                "rho_rel_l2": (s[2] / s[3]) ** 0.5 if s[3] > 0 else s[2] ** 0.5, "sums": s}
+
     def reset_stats(self) -> None:
         check(self.lib.lbm355_case_reset_stats(self.h), "reset_stats")
 

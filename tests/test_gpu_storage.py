@@ -88,3 +88,29 @@ def test_storage_against_fp64_oracle(gpu_lib, reference, storage):
     assert err_split < 0.5, (name, err_split)
     # the flow moved: the comparison is not of two rest states
     assert np.abs(uref).max() > 0.01
+
+
+def test_device_compare_against_numpy(gpu_lib):
+    """lbm355_case_compare reduces the error of one case against a reference
+    of the same geometry on the device: the four sums equal the numpy sums
+    over the non-wall cells of the two cases' dense fields."""
+    from nonnewlbm_amd import lbm355
+    g, kw = _case(_abi.F16_OFFSET)
+    a = Case(g.flat(), g.dims, **kw)
+    ref = Case(g.flat(), g.dims, **dict(kw, storage=_abi.F64))
+    with pytest.raises(lbm355._abi.LbmError, match="step"):
+        a.compare(ref)
+    a.step(300)
+    ref.step(300)
+    got = a.compare(ref)
+    act = ref.field("label") != 1
+    ua = np.stack([a.field(k)[act] for k in ("vx", "vy", "vz")])
+    ub = np.stack([ref.field(k)[act] for k in ("vx", "vy", "vz")])
+    ra, rb = a.field("rho")[act], ref.field("rho")[act]
+    want = [np.sum((ua - ub) ** 2), np.sum(ub ** 2), np.sum((ra - rb) ** 2), np.sum((rb - 1) ** 2)]
+    np.testing.assert_allclose(got["sums"], want, rtol=1e-10)
+    assert 0 < got["u_rel_l2"] < 1e-3
+    other = Case(np.zeros_like(g.flat()), g.dims, **kw)  # every cell open: other labels, other bricks
+    other.step(1)
+    with pytest.raises(lbm355._abi.LbmError, match="brick index"):
+        other.compare(ref)
