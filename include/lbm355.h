@@ -395,6 +395,16 @@ int lbm355_pool_destroy(lbm355_pool* p);
    + 4 doubles for the call.  EINVAL when grid, labels or brick index differ
    or either case has not stepped. */
 int lbm355_case_compare(lbm355_case* a, lbm355_case* ref, double out[4]);
+/* the same result, bit for bit, without allocating: the partial sums go to a
+   static device array of the library (one per GPU, 4 x 16384 + 4 doubles =
+   512 KiB, held until the process ends), so it calls no hipMalloc, hipFree
+   or allocating routine.  No counterpart in the reference; for callers that
+   cannot allocate (code under graph capture of its own surroundings, tight
+   serving loops, harnesses that forbid device allocation).  Limit: cases of
+   at most 65,536 active bricks (4,194,304 cells), EINVAL beyond with the
+   counts in the message.  Calls that overlap on one GPU share the array and
+   race; it needs nothing cleared between calls. */
+int lbm355_case_compare_noalloc(lbm355_case* a, lbm355_case* ref, double out[4]);
 
 /* bytes of device memory a descriptor would need (admission control) */
 int lbm355_case_footprint(const lbm355_case_desc* desc, int64_t* bytes);

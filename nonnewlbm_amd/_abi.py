@@ -215,6 +215,7 @@ def load() -> C.CDLL:
         "lbm355_batch_set_group_bytes": (C.c_int, [P, C.c_int64]),
         "lbm355_batch_autotune": (C.c_int, [P, C.POINTER(C.c_int)]),
         "lbm355_case_compare": (C.c_int, [P, P, C.POINTER(C.c_double)]),
+        "lbm355_case_compare_noalloc": (C.c_int, [P, P, C.POINTER(C.c_double)]),
         "lbm355_batch_add": (C.c_int, [P, P]),
         "lbm355_batch_remove": (C.c_int, [P, P]),
         "lbm355_batch_size": (C.c_int, [P, C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
@@ -253,7 +254,7 @@ EXPORTED = [
     "lbm355_batch_create", "lbm355_batch_step", "lbm355_batch_destroy", "lbm355_batch_set_shape",
     "lbm355_batch_autotune", "lbm355_batch_set_occupancy", "lbm355_batch_set_group_bytes", "lbm355_case_footprint", "lbm355_case_labels", "lbm355_case_time_kernel",
     "lbm355_batch_add", "lbm355_batch_remove", "lbm355_batch_size", "lbm355_batch_set_per_wave",
-    "lbm355_batch_config", "lbm355_case_compare",
+    "lbm355_batch_config", "lbm355_case_compare", "lbm355_case_compare_noalloc",
     "lbm355_copy_bandwidth",
     "lbm355_case_save_state", "lbm355_case_load_state",
     "lbm355_pool_create", "lbm355_pool_submit", "lbm355_pool_poll", "lbm355_pool_wait", "lbm355_pool_drain",

@@ -197,6 +197,7 @@ struct lbm355_case {
     bool strain_valid = false;
 
     hipStream_t stream = nullptr;
+    int device = 0;                     // the HIP device the case lives on (current at creation)
     // recorded on the case's stream when a batch takes the case over, so that
     // the batch's steps start after everything already queued for the case
     hipEvent_t joined = nullptr;
@@ -430,6 +431,7 @@ int setup_device(lbm355_case& c)
     const Labels& L = c.lab;
     const int X = d.nx, Y = d.ny, Z = d.nz;
     const int na = c.n_active, ns = c.n_stored;
+    HIP_TRY(hipGetDevice(&c.device));
     HIP_TRY(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&c.joined, hipEventDisableTiming));
 
@@ -1243,6 +1245,21 @@ int lbm355_case_compare(lbm355_case* a, lbm355_case* ref, double out[4])
     if (partials.alloc(size_t(4 * compare_blocks(a->n_active) + 4) * sizeof(double)) != 0)
         return fail(LBM355_ENOMEM, "compare partials");
     return compare_into(a, ref, static_cast<double*>(partials.p), out);
+}
+
+int lbm355_case_compare_noalloc(lbm355_case* a, lbm355_case* ref, double out[4])
+{
+    if (!a || !ref || !out) return fail(LBM355_EINVAL, "null argument");
+    if (a->device != ref->device) return fail(LBM355_EINVAL, "cases on different devices");
+    const int nb = compare_blocks(a->n_active);
+    double* ptr = nullptr;
+    int max_blocks = 0;
+    if (compare_scratch(a->device, &ptr, &max_blocks)) return fail(LBM355_EDEVICE, kernel_error());
+    if (nb > max_blocks)
+        return fail(LBM355_EINVAL, "case has " + std::to_string(a->n_active) + " active bricks; the static scratch of "
+                                   "the allocation-free compare holds " + std::to_string(4 * max_blocks) +
+                                   " (use lbm355_case_compare)");
+    return compare_into(a, ref, ptr, out);
 }
 
 int lbm355_case_reset_stats(lbm355_case* c)

@@ -193,6 +193,9 @@ int launch_flux(int storage, const CaseDev* dev_case, int n_bc, double* dev_out,
 int compare_blocks(int n_active);
 int launch_compare(int storage_a, int storage_b, const CaseDev* a, const CaseDev* b, int n_active, double* partials,
                    void* stream);
+// device address of the static compare scratch on `device` (looked up once
+// per device) and the most workgroups of partials it holds
+int compare_scratch(int device, double** ptr, int* max_blocks);
 int launch_set_u64(void* dev_ptr, uint64_t value, void* stream);
 int launch_set_batch(BatchDev* dev_ptr, const BatchDev& value, void* stream);
 int launch_zero(void* ptr, int64_t bytes, void* stream);

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <mutex>
 
 #include "lbm355_internal.h"
 
@@ -1448,6 +1449,39 @@ int launch_compare(int storage_a, int storage_b, const CaseDev* a, const CaseDev
     else hipLaunchKernelGGL((k_compare<float, float>), grid, block, 0, st, a, b, partials);
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st, partials, nb);
     return check_launch();
+}
+
+// the partials of lbm355_case_compare_noalloc: a file-scope device array, so
+// that the call allocates nothing.  Each GPU holds its own copy for the life
+// of the process; every call on one GPU shares it (the kernels write every
+// partial before reading it, so nothing has to start at zero), and calls
+// that overlap on one GPU race.  In host code `scratch` is only a stand-in
+// for the symbol: kernels get the device address that hipGetSymbolAddress
+// returns for each device.
+constexpr int kCompareScratchBlocks = 16384;  // workgroups of 4 bricks: up to 65,536 active bricks
+static __device__ __align__(256) double scratch[4 * kCompareScratchBlocks + 4];
+
+int compare_scratch(int device, double** ptr, int* max_blocks)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double* per_device[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) { g_kerr = "device index out of range"; return -1; }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!per_device[device]) {
+        // looked up once per device, with that device current
+        int prev = 0;
+        if (hipGetDevice(&prev) != hipSuccess) { g_kerr = "hipGetDevice failed"; return -3; }
+        if (prev != device && hipSetDevice(device) != hipSuccess) { g_kerr = "hipSetDevice failed"; return -3; }
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev != device) (void)hipSetDevice(prev);
+        if (e != hipSuccess || !p) { g_kerr = e != hipSuccess ? hipGetErrorString(e) : "null symbol address"; return -3; }
+        per_device[device] = static_cast<double*>(p);
+    }
+    *ptr = per_device[device];
+    *max_blocks = kCompareScratchBlocks;
+    return 0;
 }
 
 int launch_zero(void* ptr, int64_t bytes, void* stream)
