@@ -16,8 +16,14 @@ over its average launch duration (HIP events around back-to-back launches on
 the case's stream), against 8 TB/s; the same kernel on the 2x
 scaled junction (512x384x256, 2.9M fluid cells, ~0.7 GB of populations,
 well above the 256 MB Infinity Cache) is reported as the HBM-resident line.
-The CPU baseline is the oracle (a dense A/B OpenMP restatement of the
-reference's Host build) on a bounded number of steps of the same case.
+Also reported: each collision model on the same case, a 15-junction batch
+against the same cases one after another, the first --sweep-requests
+requests of the 200-request serving stream (Poisson arrivals at 2/s, a
+240 GB budget) against plain FP32 service of the same requests, and the
+64x32x32 FP64 channel for its full 1,000 steps on the GPU and the CPU.  The
+CPU baseline is the oracle (a dense A/B OpenMP restatement of the
+reference's Host build) on --cpu-steps steps of the benchmark case, with the
+thread count used and lscpu's physical core count.
 """
 from __future__ import annotations
 
@@ -65,52 +71,19 @@ def make_case(scale: float, storage: int, collision: int, lbm_viscosity: float, 
     return g, kw
 
 
-def sweep_requests(n: int, rate: float, seed: int = 0) -> list[dict]:
-    """Synthetic serving sweep: TCPC junctions at scales 0.5 / 0.75 / 1.0,
-    every collision model, Newtonian / CY-A / CY-B, FP32 or FP16-offset
-    storage, parabolic inflows u_max ~ U[0.01, 0.06], 1000-6000 steps,
-    Poisson arrivals at `rate` per second (numpy default_rng(seed))."""
-    from nonnewlbm_amd import _abi
-    rng = np.random.default_rng(seed)
-    t = 0.0
-    out = []
-    for k in range(n):
-        t += float(rng.exponential(1.0 / rate))
-        out.append(dict(arrival=t, scale=float(rng.choice([0.5, 0.75, 1.0])),
-                        collision=int(rng.choice([_abi.BGK, _abi.MRT, _abi.KBC, _abi.CUM])),
-                        material=str(rng.choice(["blood-newtonian", "cy-a", "cy-b"])),
-                        storage=int(rng.choice([_abi.F32, _abi.F16_OFFSET])),
-                        steps=int(rng.integers(1000, 6001)),
-                        umax=(float(rng.uniform(0.01, 0.06)), float(rng.uniform(0.01, 0.06)))))
-    return out
-
-
-def _sweep_case(req: dict, geoms: dict, storage=None):
-    from nonnewlbm_amd import _abi
-    from nonnewlbm_amd.geometry import tcpc_mask
-    from nonnewlbm_amd.materials import case_kwargs, material_by_name
-    sc = req["scale"]
-    if sc not in geoms:
-        geoms[sc] = tcpc_mask(sc)
-    g = geoms[sc]
-    kw = case_kwargs(material_by_name(req["material"]), g.dl, 0.01)
-    kw.update(storage=req["storage"] if storage is None else storage, collision=req["collision"],
-              profile=_abi.PROFILE_PARABOLIC, ivc_umax=req["umax"][0], svc_umax=req["umax"][1],
-              ivc_center=g.meta["ivc"]["center"], svc_center=g.meta["svc"]["center"],
-              ivc_radius=g.meta["ivc"]["radius"], svc_radius=g.meta["svc"]["radius"], stats_every=1)
-    return g, kw
-
-
 FULL_GRIDS = ((128, 128, 128), (192, 144, 96), (256, 192, 128), (384, 288, 192), (512, 384, 256))
+STREAM_BUDGET = 240 * 10**9
 
 
-def sweep_full_requests(n: int = 200, rate: float = 2.0, seed: int = 0) -> list[dict]:
-    """The full serving sweep: Poisson arrivals at `rate` per second; IVC
-    radius R ~ U[10, 28] cells (SVC 0.8 R, pulmonary 0.7 R, the 20/16/14 ratios
-    of the 256x192x128 junction), caval offset ~ U[0, 12] cells, grid from
-    FULL_GRIDS (dl = 64 mm / X), collision in BGK/MRT/KBC/CUM, material in
-    Newtonian/CY-A/CY-B, FP32 or FP16-offset, 5,000-50,000 steps, parabolic
-    u_max ~ U[0.01, 0.06] (numpy default_rng(seed))."""
+def stream_requests(n: int = 200, rate: float = 2.0, seed: int = 0) -> list[dict]:
+    """The serving stream: Poisson arrivals at `rate` per second; IVC radius
+    R ~ U[10, 28] cells (SVC 0.8 R, pulmonary 0.7 R, the 20/16/14 ratios of the
+    256x192x128 junction), caval offset ~ U[0, 12] cells, grid from FULL_GRIDS
+    (128^3 .. 512x384x256, dl = 64 mm / X), collision in BGK/MRT/KBC/CUM,
+    material in Newtonian/CY-A/CY-B, FP32 or FP16-offset storage,
+    5,000-50,000 steps, parabolic u_max ~ U[0.01, 0.06] (numpy
+    default_rng(seed)).  The first k requests of the stream are the same for
+    every n >= k."""
     from nonnewlbm_amd import _abi
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -127,8 +100,8 @@ def sweep_full_requests(n: int = 200, rate: float = 2.0, seed: int = 0) -> list[
     return out
 
 
-def _full_case(req: dict, storage=None):
-    """(int8 x-fastest wall mask, dims, case keywords) of a full-sweep request."""
+def stream_case(req: dict, storage=None):
+    """(int8 x-fastest wall mask, dims, case keywords) of a stream request."""
     from nonnewlbm_amd import _abi
     from nonnewlbm_amd.geometry import tcpc_mask
     from nonnewlbm_amd.materials import case_kwargs, material_by_name
@@ -143,26 +116,32 @@ def _full_case(req: dict, storage=None):
     return g.flat().astype(np.int8), g.dims, kw
 
 
-def sweep(n: int, rate: float, tick: int, full: bool = False, budget: int = 0, seq_sample: int = 0) -> dict:
-    """Requests arrive on their Poisson schedule into one Pool; latency =
-    completion - submission on the pool clock.  Baseline: the same requests
-    run back to back, one plain FP32 case at a time (all of them, or the first
-    `seq_sample` for at most 4,000 steps each)."""
+def _pcts(a: np.ndarray) -> dict:
+    return {k: round(float(np.percentile(a, q)), 3) for k, q in (("p50", 50), ("p95", 95), ("p99", 99))} | \
+        {"max": round(float(a.max()), 3)}
+
+
+def sweep(n: int, rate: float, tick: int, budget: int = STREAM_BUDGET) -> dict:
+    """The first n requests of the serving stream arrive on their Poisson
+    schedule into one Pool (continuous batching, `budget` bytes of device
+    memory); latency = completion - submission on the pool clock.
+
+    Baseline: plain FP32 service of the same n requests, one case at a time
+    with the single-case kernel.  Each request's FP32 run is timed on the GPU;
+    a FIFO queue over the same arrival times then gives the sequential
+    service's latencies and makespan (request k starts at max(arrival_k,
+    end_{k-1})).  Aggregate fluid MLUPS = fluid-cell updates of all n
+    requests / makespan (first arrival to last completion); "busy" divides by
+    the seconds the pool spent stepping instead."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from nonnewlbm_amd import _abi
     from nonnewlbm_amd.lbm355 import Case, Pool
-    if full:
-        reqs = sweep_full_requests(n, rate)
-        specs = []
-        for k, r in enumerate(reqs):
-            specs.append(_full_case(r))
-            if (k + 1) % 20 == 0:
-                print(f"# sweep: {k + 1}/{n} geometries", file=sys.stderr, flush=True)
-    else:
-        reqs = sweep_requests(n, rate)
-        geoms: dict = {}
-        specs = []
-        for r in reqs:
-            g, kw = _sweep_case(r, geoms)
-            specs.append((g.flat(), g.dims, kw))
+    reqs = stream_requests(n, rate)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        specs = list(ex.map(stream_case, reqs))
+    print(f"# sweep: {n} geometries in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     pool = Pool(budget_bytes=budget, steps_per_tick=tick, graph=True)
     t0 = time.perf_counter()
     ids = []
@@ -171,7 +150,7 @@ def sweep(n: int, rate: float, tick: int, full: bool = False, budget: int = 0, s
         if wait > 0:
             time.sleep(wait)
         ids.append(pool.submit(flat, dims, r["steps"], report_every=1000, **kw))
-    while not pool.drain(timeout=30.0):
+    while not pool.drain(timeout=20.0):
         info = pool.stats()
         print(f"# sweep: {info['done']} done, {info['running']} running, {info['queued']} queued, "
               f"{time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
@@ -183,44 +162,50 @@ def sweep(n: int, rate: float, tick: int, full: bool = False, budget: int = 0, s
     first = np.array([s["t_first_report"] - s["t_submit"] for s in done])
     span = max(s["t_done"] for s in done) - min(s["t_submit"] for s in done)
     updates = info["cell_updates"]
-    print(f"# sweep: {len(done)}/{n} done, p50 {np.percentile(lat, 50):.3f} s, p99 {np.percentile(lat, 99):.3f} s, "
-          f"{updates / info['busy_seconds'] / 1e6:.0f} fluid MLUPS while stepping", file=sys.stderr, flush=True)
-    # sequential plain FP32 baseline over the same requests (or a sample)
-    from nonnewlbm_amd import _abi
-    seq_s = 0.0
+    print(f"# sweep: {len(done)}/{n} done, p50 {np.percentile(lat, 50):.2f} s, p99 {np.percentile(lat, 99):.2f} s, "
+          f"makespan {span:.1f} s, {updates / span / 1e6:.0f} fluid MLUPS over the makespan", file=sys.stderr,
+          flush=True)
+    # plain FP32 service, one request at a time
+    seq_t = []
     seq_updates = 0
-    chosen = reqs if seq_sample <= 0 else reqs[:seq_sample]
-    for r, (flat, dims, kw) in zip(chosen, specs):
-        kw = dict(kw, storage=_abi.F32)
-        c = Case(flat, dims, **kw)
-        steps = r["steps"] if seq_sample <= 0 else min(r["steps"], 4000)
+    for k, (r, (flat, dims, kw)) in enumerate(zip(reqs, specs)):
+        c = Case(flat, dims, **dict(kw, storage=_abi.F32))
         c.sync()
         t1 = time.perf_counter()
-        c.step(steps)
+        c.step(r["steps"])
         c.sync()
-        seq_s += time.perf_counter() - t1
-        seq_updates += c.info()["active_cells"] * steps
+        seq_t.append(time.perf_counter() - t1)
+        seq_updates += c.info()["active_cells"] * r["steps"]
         c.close()
-    workload = ("TCPC junctions, IVC R ~ U[10,28], caval offset ~ U[0,12], grids 128^3 .. 512x384x256, "
-                "BGK/MRT/KBC/CUM, Newtonian/CY-A/CY-B, FP32/FP16-offset, parabolic u_max U[0.01,0.06], "
-                "5000-50000 steps" if full else
-                "TCPC junctions at scale 0.5/0.75/1.0 (128x96x64 .. 256x192x128), BGK/MRT/KBC/CUM, "
-                "Newtonian/CY-A/CY-B, FP32/FP16-offset, parabolic u_max U[0.01,0.06], 1000-6000 steps")
+        if (k + 1) % 16 == 0:
+            print(f"# sweep: sequential FP32 {k + 1}/{n}", file=sys.stderr, flush=True)
+    end = 0.0
+    seq_lat = []
+    for r, dt in zip(reqs, seq_t):
+        end = max(end, r["arrival"]) + dt
+        seq_lat.append(end - r["arrival"])
+    seq_span = end - reqs[0]["arrival"]
+    seq_busy = sum(seq_t)
+    mk = updates / span / 1e6
+    seq_mk = seq_updates / seq_span / 1e6
+    seq_rate = seq_updates / seq_busy / 1e6
     return {
-        "requests": n, "done": len(done), "failed": n - len(done), "arrival_rate_per_s": rate,
-        "steps_per_tick": tick, "seed": 0, "budget_bytes": info["budget_bytes"],
-        "workload": workload,
-        "sequential_sample": "all requests" if seq_sample <= 0 else
-                             f"the first {seq_sample} requests, at most 4000 steps each",
-        "latency_s": {"p50": round(float(np.percentile(lat, 50)), 4), "p95": round(float(np.percentile(lat, 95)), 4),
-                      "p99": round(float(np.percentile(lat, 99)), 4), "max": round(float(lat.max()), 4)},
-        "first_report_s": {"p50": round(float(np.percentile(first, 50)), 4),
-                           "p99": round(float(np.percentile(first, 99)), 4)},
+        "workload": f"the first {n} requests of the 200-request serving stream: TCPC junctions, IVC R ~ U[10,28], "
+                    "caval offset ~ U[0,12], grids 128^3 .. 512x384x256, BGK/MRT/KBC/CUM, Newtonian/CY-A/CY-B, "
+                    "FP32/FP16-offset, parabolic u_max U[0.01,0.06], 5000-50000 steps, Poisson arrivals",
+        "requests": n, "done": len(done), "failed": n - len(done), "arrival_rate_per_s": rate, "seed": 0,
+        "steps_per_tick": tick, "budget_bytes": info["budget_bytes"],
+        "last_arrival_s": round(reqs[-1]["arrival"], 3),
+        "latency_s": _pcts(lat),
+        "first_report_s": _pcts(first),
         "makespan_s": round(span, 3),
+        "fluid_mlups_makespan": round(mk, 1),
         "fluid_mlups_busy": round(updates / info["busy_seconds"] / 1e6, 1),
-        "fluid_mlups_makespan": round(updates / span / 1e6, 1),
-        "sequential_fp32_fluid_mlups": round(seq_updates / seq_s / 1e6, 1),
-        "sequential_fp32_seconds": round(seq_s, 3),
+        "sequential_fp32": {"latency_s": _pcts(np.array(seq_lat)), "makespan_s": round(seq_span, 3),
+                            "fluid_mlups_makespan": round(seq_mk, 1), "fluid_mlups_busy": round(seq_rate, 1),
+                            "gpu_seconds": round(seq_busy, 3)},
+        "speedup_makespan_mlups": round(mk / seq_mk, 3),
+        "speedup_vs_sequential_busy_mlups": round(mk / seq_rate, 3),
     }
 
 
@@ -428,23 +413,117 @@ def configure(case, launch: str) -> tuple[int, int, int]:
     return bpg, nt, cap
 
 
-def cpu_baseline(g, kw, budget_s: float) -> dict:
-    from oracle.oracle import OracleCase
+def host_cpus() -> dict:
+    """Host CPU of this process: lscpu's model and physical cores (sockets x
+    cores per socket), and the CPUs this process may run on."""
+    import subprocess
+    out = {"model": None, "physical_cores": None, "threads_per_core": None,
+           "cpus_available": len(os.sched_getaffinity(0))}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return out
+    kv = {}
+    for line in txt.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    out["model"] = kv.get("Model name")
+    try:
+        out["physical_cores"] = int(kv["Socket(s)"]) * int(kv["Core(s) per socket"])
+        out["threads_per_core"] = int(kv["Thread(s) per core"])
+    except (KeyError, ValueError):
+        pass
+    return out
 
+
+def channel_case(storage: int):
+    """The 64x32x32 channel: radius 12 centred in y-z, parabolic inflow
+    u_max = 0.05 at x-min, pressure outlet (rho = 1) at x-max, cumulant,
+    Newtonian tau = 0.6 (nu = 1/30)."""
+    from nonnewlbm_amd import _abi
+    from nonnewlbm_amd.geometry import channel_mask
+    g = channel_mask()
+    kw = dict(storage=storage, collision=_abi.CUM, nu_ref=(0.6 - 0.5) / 3.0, profile=_abi.PROFILE_PARABOLIC,
+              ivc_umax=0.05, ivc_radius=12.0, ivc_center=(15.5, 15.5), face_bc=_abi.CHANNEL_FACES,
+              dl=1.0e-3, dt=1.0e-3)
+    return g, kw
+
+
+def _timed_oracle(g, kw, steps: int) -> tuple:
+    from oracle.oracle import OracleCase
     o = OracleCase(g.flat(), g.dims, **kw)
-    active = o.info()["active_cells"]
-    t0 = time.perf_counter()
     o.step(1)
-    one = time.perf_counter() - t0
-    n = max(1, min(200, int(budget_s / max(one, 1e-3))))
     t0 = time.perf_counter()
-    o.step(n)
-    dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(active * n / dt / 1e6, 3), "unit": "fluid MLUPS", "cores": cores, "kind": "port",
-            "all_sites_mlups": round(int(np.prod(g.dims)) * n / dt / 1e6, 3),
-            "sample": f"{n} steps of the same case (after 1 warm-up step), {dt:.1f} s, dense A/B over "
-                      f"{int(np.prod(g.dims))} cells"}
+    o.step(steps)
+    return o, time.perf_counter() - t0
+
+
+def _rel(a, b) -> float:
+    return float(np.sqrt(np.sum((a - b) ** 2)) / max(np.sqrt(np.sum(b ** 2)), 1e-300))
+
+
+def channel_line() -> dict:
+    """The 64x32x32 channel (cumulant, FP64, tau = 0.6) for its full 1,000
+    steps on the GPU and on the CPU oracle, with the relative L2 of u and of
+    rho - 1 between the two runs."""
+    from nonnewlbm_amd import _abi
+    from nonnewlbm_amd.lbm355 import Case
+    from oracle import oracle
+    g, kw = channel_case(_abi.F64)
+    c = Case(g.flat(), g.dims, **kw)
+    c.step(1)
+    c.sync()
+    t0 = time.perf_counter()
+    c.step(1000)
+    c.sync()
+    gpu_s = time.perf_counter() - t0
+    o, cpu_s = _timed_oracle(g, kw, 1000)
+    act = o.field("label") != 1
+    n = int(act.sum())
+    u = np.stack([c.field(k)[act] for k in ("vx", "vy", "vz")])
+    uo = np.stack([o.field(k)[act] for k in ("vx", "vy", "vz")])
+    out = {"workload": "channel 64x32x32, radius 12, cumulant, FP64, tau=0.6, parabolic inflow u_max=0.05, "
+                       "1000 steps (after one untimed step)",
+           "fluid_cells": n,
+           "gpu_fluid_mlups": round(n * 1000 / gpu_s / 1e6, 2), "gpu_seconds": round(gpu_s, 4),
+           "cpu_fluid_mlups": round(n * 1000 / cpu_s / 1e6, 3), "cpu_seconds": round(cpu_s, 3),
+           "cpu_threads": oracle.threads(),
+           "u_rel_l2_gpu_vs_cpu": _rel(u, uo),
+           "rho_rel_l2_gpu_vs_cpu": _rel(c.field("rho")[act] - 1.0, o.field("rho")[act] - 1.0)}
+    c.close()
+    print(f"# channel: GPU {out['gpu_fluid_mlups']} / CPU {out['cpu_fluid_mlups']} fluid MLUPS, "
+          f"u rel L2 {out['u_rel_l2_gpu_vs_cpu']:.2e}", file=sys.stderr, flush=True)
+    return out
+
+
+def cpu_baseline(g, kw, steps: int, all_models: bool) -> dict:
+    """The oracle (a dense A/B OpenMP restatement of the reference's Host
+    build, populations held in the case's storage type) on `steps` steps of
+    the benchmark case after one untimed step, on every thread OpenMP is
+    given; with all_models also BGK, MRT_LES and KBC_N4 on the same steps."""
+    from nonnewlbm_amd import _abi
+    from oracle import oracle
+    o, dt = _timed_oracle(g, kw, steps)
+    active = o.info()["active_cells"]
+    del o
+    host = host_cpus()
+    out = {"value": round(active * steps / dt / 1e6, 3), "unit": "fluid MLUPS", "cores": oracle.threads(),
+           "kind": "port",
+           "all_sites_mlups": round(int(np.prod(g.dims)) * steps / dt / 1e6, 3),
+           "sample": f"{steps} steps of the benchmark case (after 1 untimed step), {dt:.1f} s, dense A/B over "
+                     f"{int(np.prod(g.dims))} cells, {oracle.threads()} OpenMP threads",
+           "host": host}
+    if all_models:
+        rates = {_abi.COLLISION_IDS[kw["collision"]]: out["value"]}
+        for coll in (_abi.BGK, _abi.MRT, _abi.KBC):
+            if coll == kw["collision"]:
+                continue
+            o, dt = _timed_oracle(g, dict(kw, collision=coll), steps)
+            rates[_abi.COLLISION_IDS[coll]] = round(active * steps / dt / 1e6, 3)
+            del o
+        out["models"] = rates
+    return out
 
 
 def main() -> int:
@@ -459,8 +538,10 @@ def main() -> int:
                     help="Carreau-Yasuda coupling of the workload (fd-force adds the reference's pre-kernels)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-hbm", action="store_true", help="skip the 512x384x256 HBM-resident line")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-sweep", action="store_true", help="skip the serving sweep")
+    ap.add_argument("--cpu-steps", type=int, default=500, help="timed steps of the CPU baseline")
+    ap.add_argument("--cpu-models", action="store_true", help="CPU baseline for all four collision models")
+    ap.add_argument("--no-channel", action="store_true", help="skip the 64x32x32 channel GPU/CPU line")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the serving stream")
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
                     help="also run the FP64/FP32/FP16/BF16 storage study for STEPS steps")
     ap.add_argument("--precision-scale", type=float, default=2.0)
@@ -473,10 +554,9 @@ def main() -> int:
                     help="steps of the 15-junction batch against sequential stepping (0: skip)")
     ap.add_argument("--model-steps", type=int, default=2000,
                     help="steps per collision model on the 256x192x128 workload (0: skip)")
-    ap.add_argument("--sweep-full", action="store_true",
-                    help="the 200-request sweep at its real sizes (several minutes) instead of the short one")
-    ap.add_argument("--sweep-requests", type=int, default=48)
-    ap.add_argument("--sweep-rate", type=float, default=12.0, help="Poisson arrivals per second")
+    ap.add_argument("--sweep-requests", type=int, default=48,
+                    help="serve the first N requests of the 200-request stream (200: all of it)")
+    ap.add_argument("--sweep-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--sweep-tick", type=int, default=64, help="steps per pool tick")
     args = ap.parse_args()
 
@@ -625,16 +705,17 @@ def main() -> int:
     if args.batch_steps > 0:
         out["batch15"] = batch15(args.batch_steps)
 
-    if args.sweep_full:
-        out["sweep_full"] = sweep(200, 2.0, args.sweep_tick, full=True, budget=240 * 10**9, seq_sample=16)
-    elif not args.no_sweep:
+    if not args.no_sweep:
         out["sweep"] = sweep(args.sweep_requests, args.sweep_rate, args.sweep_tick)
 
     if args.precision_study > 0:
         out["precision_study"] = precision_study(args.precision_scale, args.precision_study)
 
+    if not args.no_channel:
+        out["channel"] = channel_line()
+
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(g, kw, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(g, kw, args.cpu_steps, args.cpu_models)
         print(f"# cpu baseline {out['cpu_baseline']}", file=sys.stderr, flush=True)
 
     print(json.dumps(out), flush=True)

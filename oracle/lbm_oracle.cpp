@@ -34,6 +34,8 @@
 #include <string>
 #include <vector>
 
+#include <omp.h>
+
 #include "../include/lbm355.h"
 
 namespace {
@@ -892,6 +894,9 @@ struct Oracle : OracleBase {
 extern "C" {
 
 const char* oracle_last_error(void) { return g_err.c_str(); }
+
+// OpenMP threads that a step uses (OMP_NUM_THREADS, else every online CPU)
+int oracle_threads(void) { return omp_get_max_threads(); }
 
 void* oracle_create(const lbm355_case_desc* d)
 {

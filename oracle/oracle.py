@@ -53,8 +53,14 @@ def load() -> C.CDLL:
     lib.oracle_neq_nu_from_G.restype = C.c_double
     lib.oracle_neq_nu_from_G.argtypes = [C.c_double] * 6
     lib.oracle_last_error.restype = C.c_char_p
+    lib.oracle_threads.restype = C.c_int
     _lib = lib
     return lib
+
+
+def threads() -> int:
+    """OpenMP threads of an oracle step."""
+    return int(load().oracle_threads())
 
 
 def _p(a: np.ndarray):
