@@ -119,9 +119,12 @@ def _coords(n: int, dl: float) -> np.ndarray:
     return (np.arange(n, dtype=np.float64) * dl).astype(">f4")
 
 
-def write_wall_vtk(path: str | Path, geom: Geometry, binary: bool = True) -> None:
+def write_wall_vtk(path: str | Path, geom: Geometry, binary: bool = True, coords: str = "float") -> None:
     """Write the wall mask as the reference's geometry.vtk: RECTILINEAR_GRID
-    with point coordinates 0..N*dl and CELL_DATA ``wall`` (double)."""
+    with point coordinates 0..N*dl (``coords`` "float" or "double") and
+    CELL_DATA ``wall`` (double)."""
+    if coords not in ("float", "double"):
+        raise ValueError(f"coords must be 'float' or 'double', not {coords!r}")
     X, Y, Z = geom.dims
     path = Path(path)
     with path.open("wb") as f:
@@ -131,8 +134,8 @@ def write_wall_vtk(path: str | Path, geom: Geometry, binary: bool = True) -> Non
         f.write(b"DATASET RECTILINEAR_GRID\n")
         f.write(f"DIMENSIONS {X + 1} {Y + 1} {Z + 1}\n".encode())
         for name, n in (("X", X), ("Y", Y), ("Z", Z)):
-            c = _coords(n + 1, geom.dl)
-            f.write(f"{name}_COORDINATES {n + 1} float\n".encode())
+            c = (np.arange(n + 1, dtype=np.float64) * geom.dl).astype(">f8" if coords == "double" else ">f4")
+            f.write(f"{name}_COORDINATES {n + 1} {coords}\n".encode())
             if binary:
                 f.write(c.tobytes())
                 f.write(b"\n")
