@@ -121,7 +121,7 @@ def _pcts(a: np.ndarray) -> dict:
         {"max": round(float(a.max()), 3)}
 
 
-def sweep(n: int, rate: float, tick: int, budget: int = STREAM_BUDGET) -> dict:
+def sweep(n: int, rate: float, tick: int, budget: int = STREAM_BUDGET, graph: bool = True) -> dict:
     """The first n requests of the serving stream arrive on their Poisson
     schedule into one Pool (continuous batching, `budget` bytes of device
     memory); latency = completion - submission on the pool clock.
@@ -142,7 +142,7 @@ def sweep(n: int, rate: float, tick: int, budget: int = STREAM_BUDGET) -> dict:
     with ThreadPoolExecutor(max_workers=8) as ex:
         specs = list(ex.map(stream_case, reqs))
     print(f"# sweep: {n} geometries in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    pool = Pool(budget_bytes=budget, steps_per_tick=tick, graph=True)
+    pool = Pool(budget_bytes=budget, steps_per_tick=tick, graph=graph)
     t0 = time.perf_counter()
     ids = []
     for r, (flat, dims, kw) in zip(reqs, specs):
@@ -209,7 +209,7 @@ def sweep(n: int, rate: float, tick: int, budget: int = STREAM_BUDGET) -> dict:
     }
 
 
-def batch15(steps: int) -> dict:
+def batch15(steps: int, graph: bool = True, small: bool = True) -> dict:
     """Fifteen independent junctions stepped as one batch (one launch per
     step over their concatenated brick lists, graph replay) against the same
     cases stepped one after another, each with the plain single-case launch:
@@ -249,7 +249,7 @@ def batch15(steps: int) -> dict:
         c.close()
     # batched
     cases = [Case(g.flat(), g.dims, **kw) for sc, g, kw in specs]
-    b = Batch(cases, graph=True)
+    b = Batch(cases, graph=graph)
     b.step(2)
     t0 = time.perf_counter()
     b.step(steps)
@@ -268,18 +268,20 @@ def batch15(steps: int) -> dict:
     bat = cells * steps / bat_s / 1e6
     print(f"# batch of 15: {bat:.0f} fluid MLUPS batched, {seq:.0f} sequential ({bat / seq:.2f}x)", file=sys.stderr,
           flush=True)
-    small = small_batch(steps)
-    return {"workload": "15 TCPC junctions (5 parabolic u_max pairs ~ U[0.01,0.06] x Newtonian/CY-A/CY-B, scale "
+    out = {"workload": "15 TCPC junctions (5 parabolic u_max pairs ~ U[0.01,0.06] x Newtonian/CY-A/CY-B, scale "
                         "per case from 0.5/0.75/1.0/1.5, default_rng(0)), cumulant, FP32",
             "scales": [sc for sc, g, kw in specs], "fluid_cells": cells, "steps": steps,
             "batched_fluid_mlups": round(bat, 1), "sequential_fluid_mlups": round(seq, 1),
             "speedup": round(bat / seq, 3),
             "one_group_fluid_mlups": round(cells * steps / one_s / 1e6, 1),
             "grouping": "consecutive members up to 256 MiB of device memory per group, 64 steps per group in turn",
-            "results_finite": finite, "small_cases": small}
+            "results_finite": finite, "graph": graph}
+    if small:
+        out["small_cases"] = small_batch(steps, graph)
+    return out
 
 
-def small_batch(steps: int) -> dict:
+def small_batch(steps: int, graph: bool = True) -> dict:
     """Sixteen 128x96x64 junctions (44,420 fluid cells each, CY-A, cumulant,
     FP32, u_max pairs ~ U[0.01, 0.06], default_rng(1)): one such case leaves
     most of the GPU idle, a batch fills it."""
@@ -307,7 +309,7 @@ def small_batch(steps: int) -> dict:
         c.step(steps)
         c.sync()
         seq_s += time.perf_counter() - t0
-    b = Batch(cases, graph=True)
+    b = Batch(cases, graph=graph)
     b.step(2)
     t0 = time.perf_counter()
     b.step(steps)
@@ -558,6 +560,8 @@ def main() -> int:
                     help="serve the first N requests of the 200-request stream (200: all of it)")
     ap.add_argument("--sweep-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--sweep-tick", type=int, default=64, help="steps per pool tick")
+    ap.add_argument("--graph", type=int, default=1, choices=[0, 1],
+                    help="batches and the pool replay captured graphs (1) or launch directly (0)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -703,10 +707,10 @@ def main() -> int:
         out["models"] = models(args.model_steps)
 
     if args.batch_steps > 0:
-        out["batch15"] = batch15(args.batch_steps)
+        out["batch15"] = batch15(args.batch_steps, bool(args.graph))
 
     if not args.no_sweep:
-        out["sweep"] = sweep(args.sweep_requests, args.sweep_rate, args.sweep_tick)
+        out["sweep"] = sweep(args.sweep_requests, args.sweep_rate, args.sweep_tick, graph=bool(args.graph))
 
     if args.precision_study > 0:
         out["precision_study"] = precision_study(args.precision_scale, args.precision_study)
