@@ -374,6 +374,7 @@ def precision_study(scale: float, steps: int) -> dict:
         lab = c.field("label")
         act = lab != 1
         u = np.stack([c.field(k)[act] for k in ("vx", "vy", "vz")])
+        rho1 = c.field("rho")[act] - 1.0
         c.compute_strain()
         S = {k: c.field(k)[act] for k in ("s11", "s12", "s13", "s22", "s32", "s33")}
         gam = np.sqrt(2.0) * np.sqrt(S["s11"] ** 2 + S["s22"] ** 2 + S["s33"] ** 2
@@ -393,12 +394,12 @@ def precision_study(scale: float, steps: int) -> dict:
         row = {"seconds": round(secs, 3), "fluid_mlups": round(int(act.sum()) * steps / secs / 1e6, 1),
                "lpa_split": round(float(split), 6)}
         if ref is None:
-            ref = (u, gdot, wss, split)
+            ref = (u, gdot, wss, split, rho1)
         else:
             def rl2(a, b):
                 return float(np.sqrt(np.sum((a - b) ** 2)) / np.sqrt(np.sum(b ** 2)))
-            row.update(u_rel_l2=rl2(u, ref[0]), gamma_dot_rel_l2=rl2(gdot, ref[1]), wss_rel_l2=rl2(wss, ref[2]),
-                       split_abs_diff_percent=round(100 * abs(split - ref[3]), 5))
+            row.update(u_rel_l2=rl2(u, ref[0]), rho_minus_1_rel_l2=rl2(rho1, ref[4]), gamma_dot_rel_l2=rl2(gdot, ref[1]),
+                       wss_rel_l2=rl2(wss, ref[2]), split_abs_diff_percent=round(100 * abs(split - ref[3]), 5))
         res["storages"][_abi.STORAGE_NAMES[st]] = row
         print(f"# precision {_abi.STORAGE_NAMES[st]}: {row}", file=sys.stderr, flush=True)
     return res

@@ -59,7 +59,7 @@ enum {
                                16/27), stochastically rounded, FP32 arithmetic  */
     LBM355_BF16_OFFSET = 3  /* f_q - w_q stored as bfloat16, stochastically
                                rounded, FP32 arithmetic; for precision studies
-                               only: it misses the 1e-3 velocity bar (4.3e-3 on
+                               only: it misses the 1e-3 velocity bar (about 4e-3 on
                                the 512x384x256 junction after 40,000 steps), so
                                the pool and the sim_tcpc CLI refuse it          */
 };

@@ -386,7 +386,7 @@ int lbm355_pool_submit(lbm355_pool* p, const lbm355_case_desc* desc, int64_t ste
     if (!p || !desc || !id || !desc->geometry || steps <= 0) return pool_fail(LBM355_EINVAL, "bad submit arguments");
     if (desc->storage == LBM355_BF16_OFFSET)
         return pool_fail(LBM355_EINVAL, "bfloat16-offset storage is not served: its velocity error against FP64 "
-                                        "(4.3e-3 on the 512x384x256 junction) exceeds the 1e-3 bar; use FP16-offset");
+                                        "(4e-3 after 40,000 steps of the 512x384x256 junction) exceeds the 1e-3 bar; use FP16-offset");
     std::unique_ptr<Request> r(new (std::nothrow) Request());
     if (!r) return pool_fail(LBM355_ENOMEM, "host allocation");
     int64_t fp = 0;
