@@ -228,13 +228,30 @@ struct lbm355_case {
         y = brick_xyz[3 * b + 1] * kBY + ((l >> kSX) & (kBY - 1)) - oy;
         z = brick_xyz[3 * b + 2] * kBZ + (l >> (kSX + kSY));
     }
+    // the cell whose storage holds AA pair (cell, slot): the cell itself, or
+    // for the two-byte types and slot < 13 the cell - c_slot (balanced
+    // addressing, as the kernels' balanced_loads)
+    void pair_position(int& x, int& y, int& z, int slot) const
+    {
+        if (balanced_storage(storage) && slot < 13) { x -= qx(slot); y -= qy(slot); z -= qz(slot); }
+    }
+    // element offset of AA pair (x,y,z; slot); the dummy brick if it has no storage
     int32_t slot_offset(int x, int y, int z, int slot) const
     {
+        pair_position(x, y, z, slot);
         int32_t b = stored_brick(x, y, z);
         int lane = 0;
         if (b < 0) b = n_stored;
         else lane = lane_of(x, y, z);
         return (b * kQ + slot) * kLanes + lane;
+    }
+    // the same, or -1 if the pair has no storage (outside the grid or the
+    // stored bricks): such a pair holds a population that no step reads
+    int32_t pair_offset(int x, int y, int z, int slot) const
+    {
+        pair_position(x, y, z, slot);
+        const int32_t b = stored_brick(x, y, z);
+        return b < 0 ? -1 : (b * kQ + slot) * kLanes + lane_of(x, y, z);
     }
 };
 
@@ -1160,15 +1177,11 @@ int lbm355_case_get_populations(lbm355_case* c, double* host, size_t bytes)
                 // through full-way bounce-back; the others hold nothing
                 if (c->lab.map[i] == L_WALL && c->stored_brick(x, y, z) < 0) continue;
                 for (int q = 0; q < kQ; ++q) {
-                    int32_t off;
-                    if (even_next) {
-                        const int tx = x + qx(q), ty = y + qy(q), tz = z + qz(q);
-                        if (tx < 0 || ty < 0 || tz < 0 || tx >= X || ty >= Y || tz >= Z) continue;
-                        if (c->stored_brick(tx, ty, tz) < 0) continue;
-                        off = c->slot_offset(tx, ty, tz, q);
-                    } else {
-                        off = c->slot_offset(x, y, z, 26 - q);
-                    }
+                    // post-collision f_q of the cell: pair (x + c_q, q) after an
+                    // odd step, pair (x, opp q) after an even one
+                    const int32_t off = even_next ? c->pair_offset(x + qx(q), y + qy(q), z + qz(q), q)
+                                                  : c->pair_offset(x, y, z, 26 - q);
+                    if (off < 0) continue;
                     host[int64_t(q) * N + i] = value(off, q);
                 }
             }
@@ -1446,15 +1459,9 @@ int set_populations(lbm355_case* c, const std::vector<double>& pops)
                 const int64_t i = c->cell_index(x, y, z);
                 if (c->lab.map[i] == L_WALL && c->stored_brick(x, y, z) < 0) continue;
                 for (int q = 0; q < kQ; ++q) {
-                    int32_t off;
-                    if (even_next) {
-                        const int tx = x + qx(q), ty = y + qy(q), tz = z + qz(q);
-                        if (tx < 0 || ty < 0 || tz < 0 || tx >= X || ty >= Y || tz >= Z) continue;
-                        if (c->stored_brick(tx, ty, tz) < 0) continue;
-                        off = c->slot_offset(tx, ty, tz, q);
-                    } else {
-                        off = c->slot_offset(x, y, z, 26 - q);
-                    }
+                    const int32_t off = even_next ? c->pair_offset(x + qx(q), y + qy(q), z + qz(q), q)
+                                                  : c->pair_offset(x, y, z, 26 - q);
+                    if (off < 0) continue;
                     put(off, q, pops[size_t(int64_t(q) * N + i)]);
                 }
             }

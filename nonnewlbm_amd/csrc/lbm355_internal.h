@@ -147,6 +147,12 @@ struct StepLaunch {
 
 // storage kinds (same values as LBM355_F64 ...)
 enum Storage : int { S_F64 = 0, S_F32 = 1, S_F16 = 2, S_BF16 = 3 };
+// the two-byte storage types use the balanced AA addressing: pair (cell P,
+// slot s) lives at cell P - c_s for s < 13 and at P otherwise, so that both
+// AA steps shift half of the directions (see balanced_loads)
+template <int S>
+constexpr bool kBalanced = S == S_F16 || S == S_BF16;
+constexpr bool balanced_storage(int s) { return s == S_F16 || s == S_BF16; }
 enum Model : int { M_CUM = 0, M_BGK = 1, M_MRT = 2, M_KBC = 3 };
 
 // D3Q27 lattice weights w_q, q = 9(cx+1)+3(cy+1)+(cz+1)

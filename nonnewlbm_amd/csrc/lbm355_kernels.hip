@@ -701,6 +701,33 @@ __device__ __forceinline__ int pick_brick(const int (&t)[27], int dx, int dy, in
 
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return (uint64_t(1) << lane) - 1; }
 
+// balanced AA addressing of the two-byte storage types: the pair (cell P,
+// slot s) of the AA scheme is stored at cell P - c_s for s < 13 and at P
+// otherwise (host: lbm355_case::pair_position).  The AA steps then become
+// alike: each reads the 13 populations q < 13 one cell upstream and writes the
+// 13 populations q > 13 one cell downstream, instead of the odd step shifting
+// all 26 (512x384x256 FP16: 93.7 -> 84.1 us per step).  The loads of a
+// step: even steps read slot q, odd steps slot 26 - q
+template <int S>
+__device__ __forceinline__ void balanced_loads(const rsrc_t& F, const int (&t)[27], uint32_t own, int i, int j, int k,
+                                               int lane, int parity, typename Pop<S>::R (&f)[27])
+{
+    using P = Pop<S>;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const uint32_t sl = uint32_t(parity == 0 ? q : 26 - q) * kLanes;
+        if (q < 13) {
+            const int cx = qx(q), cy = qy(q), cz = qz(q);
+            const int si = i - cx, sj = j - cy, sk = k - cz;
+            const int nb = pick_brick(t, -cx, -cy, -cz, (si & ~(kBX - 1)) != 0, (sj & ~(kBY - 1)) != 0,
+                                      (sk & ~(kBZ - 1)) != 0);
+            f[q] = P::ld(F, uint32_t(nb + brick_lane(si, sj, sk)) + sl, q);
+        } else {
+            f[q] = P::ld(F, own + sl, q);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // the fused step
 // one time step of brick b of case C at launch index `launch` (the case's
@@ -753,7 +780,9 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
     R f[27];
     R rho = R(1), ux = R(0), uy = R(0), uz = R(0);
     if (fluid) {
-        if (parity == 0) {
+        if constexpr (kBalanced<S>) {
+            balanced_loads<S>(F, t, own, i, j, k, lane, parity, f);
+        } else if (parity == 0) {
 #pragma unroll
             for (int q = 0; q < 27; ++q) f[q] = P::template ld<kEvenAux>(F, own + uint32_t(q * kLanes), q);
         } else {
@@ -815,7 +844,23 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
         const uint32_t H = (S == S_F16 || S == S_BF16)
                                ? mix32(uint32_t(b * kLanes + lane) ^ mix32(uint32_t(step) * 0x85ebca6bu + 0x632be5abu))
                                : 0u;
-        if (parity == 0) {
+        if constexpr (kBalanced<S>) {
+            // the writes of q > 13 land one cell downstream, the others in the
+            // cell's own slots: even steps write slot 26 - q, odd steps slot q
+#pragma unroll
+            for (int q = 0; q < 27; ++q) {
+                const uint32_t sl = uint32_t(parity == 0 ? 26 - q : q) * kLanes;
+                if (q > 13) {
+                    const int cx = qx(q), cy = qy(q), cz = qz(q);
+                    const int di = i + cx, dj = j + cy, dk = k + cz;
+                    const int nb = pick_brick(t, cx, cy, cz, (di & ~(kBX - 1)) != 0, (dj & ~(kBY - 1)) != 0,
+                                              (dk & ~(kBZ - 1)) != 0);
+                    P::st(F, uint32_t(nb + brick_lane(di, dj, dk)) + sl, q, f[q], rnd_q(H, q));
+                } else {
+                    P::st(F, own + sl, q, f[q], rnd_q(H, q));
+                }
+            }
+        } else if (parity == 0) {
 #pragma unroll
             for (int q = 0; q < 27; ++q)
                 P::template st<kEvenAux>(F, own + uint32_t((26 - q) * kLanes), q, f[q], rnd_q(H, q));
@@ -919,7 +964,9 @@ __device__ __forceinline__ void prepass_brick(const StepLaunch& L, const BrickRe
     for (int q = 0; q < 27; ++q) t[q] = nbr_c[q];
     const int i = lane & (kBX - 1), j = (lane >> kSX) & (kBY - 1), k = lane >> (kSX + kSY);
     R f[27];
-    if (parity == 0) {
+    if constexpr (kBalanced<S>) {
+        balanced_loads<S>(F, t, uint32_t(t[13]) + uint32_t(lane), i, j, k, lane, parity, f);
+    } else if (parity == 0) {
         const uint32_t own = uint32_t(t[13]) + uint32_t(lane);
 #pragma unroll
         for (int q = 0; q < 27; ++q) f[q] = P::ld(F, own + uint32_t(q * kLanes), q);

@@ -28,7 +28,7 @@ def tcpc_case(scale=0.25, storage=_abi.F32, collision=_abi.CUM, **extra):
     return g, kw
 
 
-def stored_populations(act: np.ndarray, steps: int) -> np.ndarray:
+def stored_populations(act: np.ndarray, steps: int, balanced: bool = False) -> np.ndarray:
     """[27, n] mask over the populations of the active cells ``act`` (in the
     order of ``pops[:, act]``) that the AA layout holds after ``steps`` steps.
 
@@ -38,12 +38,18 @@ def stored_populations(act: np.ndarray, steps: int) -> np.ndarray:
     where that cell is outside the grid the population has no slot.  Those
     are exactly the outward directions of the cells on the grid faces, which
     no step reads (the clamped pull and the boundary partner reads only take
-    inward and tangential directions), and the library reports them as 0."""
+    inward and tangential directions), and the library reports them as 0.
+
+    ``balanced``: the layout of the two-byte storage types, where pair (P, s)
+    is stored at P - c_s for s < 13.  After any number of steps population q
+    of cell x is then held unless q > 13 and x + c_q is outside the grid."""
     idx = np.nonzero(act)
     held = np.ones((27, idx[0].size), dtype=bool)
-    if steps % 2 == 1:
+    if steps % 2 == 1 and not balanced:
         return held
     for q in range(27):
+        if balanced and q <= 13:
+            continue
         c = (q // 9 - 1, (q // 3) % 3 - 1, q % 3 - 1)
         for a in range(3):
             t = idx[a] + c[a]
@@ -51,10 +57,11 @@ def stored_populations(act: np.ndarray, steps: int) -> np.ndarray:
     return held
 
 
-def assert_populations_match(pg: np.ndarray, pc: np.ndarray, act: np.ndarray, steps: int, atol: float) -> None:
+def assert_populations_match(pg: np.ndarray, pc: np.ndarray, act: np.ndarray, steps: int, atol: float,
+                             balanced: bool = False) -> None:
     """GPU populations ``pg`` against the oracle's ``pc`` ([27, X, Y, Z]) on
     every active cell: the held ones to ``atol``, the unheld ones exactly 0."""
-    held = stored_populations(act, steps)
+    held = stored_populations(act, steps, balanced)
     a, b = pg[:, act], pc[:, act]
     assert np.all(a[~held] == 0.0), "a population without a slot is reported non-zero"
     worst = float(np.max(np.abs(a[held] - b[held])))

@@ -25,7 +25,7 @@ import pytest
 from nonnewlbm_amd import _abi
 from nonnewlbm_amd.lbm355 import Case
 from oracle.oracle import OracleCase
-from tests.helpers import rel_l2
+from tests.helpers import assert_populations_match, channel_case, rel_l2, tcpc_case
 
 pytestmark = pytest.mark.gpu
 
@@ -114,3 +114,24 @@ def test_device_compare_against_numpy(gpu_lib):
     other.step(1)
     with pytest.raises(lbm355._abi.LbmError, match="brick index"):
         other.compare(ref)
+
+
+@pytest.mark.parametrize("storage,atol", [(_abi.F16_OFFSET, 5e-4), (_abi.BF16_OFFSET, 5e-3)])
+@pytest.mark.parametrize("make", ["channel", "tcpc"])
+def test_half_storage_populations_against_oracle(gpu_lib, storage, atol, make):
+    """Two-byte storage keeps the AA pairs in the balanced layout (pair (P, s)
+    stored at P - c_s for s < 13): after 1, 2, 3, 10 and 40 steps every held
+    population equals the FP64 oracle's to the storage's rounding (FP16 5e-4,
+    BF16 5e-3 absolute), and exactly the populations q > 13 that stream out
+    of the grid are unheld and reported 0.  The parabolic channel and the
+    junction with plug inflows cover the inflow, outflow and outlet planes."""
+    g, kw = channel_case(storage) if make == "channel" else tcpc_case(0.25, storage)
+    gpu = Case(g.flat(), g.dims, **kw)
+    cpu = OracleCase(g.flat(), g.dims, **dict(kw, storage=_abi.F64))
+    act = cpu.field("label") != 1
+    done = 0
+    for n in (1, 2, 3, 10, 40):
+        gpu.step(n - done)
+        cpu.step(n - done)
+        done = n
+        assert_populations_match(gpu.populations(), cpu.populations(), act, done, atol, balanced=True)
