@@ -13,6 +13,7 @@
 #include <cstring>
 #include <sys/stat.h>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -25,6 +26,13 @@ using namespace lbm355;
 namespace {
 
 thread_local std::string g_error;
+
+// a batch's graph capture and the device-memory management of case setup and
+// teardown (hipMalloc, hipFree, synchronous copies) never overlap across
+// threads: the serving pool creates and destroys cases on a builder thread
+// while its worker captures graphs, and a device-wide synchronisation inside
+// an allocator call must not meet a stream in capture
+std::mutex g_device_mu;
 
 int fail(int code, const std::string& msg)
 {
@@ -890,7 +898,10 @@ int lbm355_case_create(const lbm355_case_desc* desc, lbm355_case** out)
         if ((rc = c->lab.build(tmp))) return rc;
     }
     if ((rc = build_index(*c))) return rc;
-    if ((rc = setup_device(*c))) return rc;
+    {
+        std::lock_guard<std::mutex> device_lock(g_device_mu);
+        if ((rc = setup_device(*c))) return rc;
+    }
     const bool term = c->lab.terminate;
     *out = c.release();
     if (term) return fail(LBM355_ETERMINATE, "zero inlet or outlet area");
@@ -899,6 +910,7 @@ int lbm355_case_create(const lbm355_case_desc* desc, lbm355_case** out)
 
 int lbm355_case_destroy(lbm355_case* c)
 {
+    std::lock_guard<std::mutex> device_lock(g_device_mu);
     delete c;
     return LBM355_OK;
 }
@@ -2092,6 +2104,7 @@ int batch_enqueue(lbm355_batch* b, int64_t n)
             // 0..chunk-1) of the group whose range the descriptor holds;
             // replayed once per group, then the launch counter advances
             StepLaunch L = batch_step_launch(*b, grid);
+            std::lock_guard<std::mutex> capture_lock(g_device_mu);
             HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
             int r = 0;
             for (int s = 0; s < chunk && r == 0; ++s) {
