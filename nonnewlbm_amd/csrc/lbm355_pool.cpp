@@ -12,7 +12,8 @@
 // re-capturing the batch's graph); batches of different types are queued on
 // their own streams in the same tick; finished cases leave their batch at the
 // end of their last tick, so that waiting ones can take their memory
-// (continuous batching).
+// (continuous batching).  Each tick steps the set of resident cases that the
+// tick scheduling below picks (shortest remaining work first, aged).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,6 +55,7 @@ struct Request {
     int64_t active_cells = 0;
     lbm355_request_status st{};
     bool admitted = false;  // its footprint is counted in the pool's used bytes
+    bool member = false;    // a member of its batch (stepped by the ticks) rather than only resident
     bool taken = false;
     int build_rc = LBM355_OK;   // result of the case creation (builder thread)
     std::string message;        // error message of a failed creation or tick
@@ -63,6 +65,24 @@ double now_s(std::chrono::steady_clock::time_point t0)
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
+
+// Tick scheduling.  Every resident case used to step in every tick, which is
+// processor sharing: with the GPU already saturated by one large case, each
+// request then finishes late (first 48 requests of the 200-request stream:
+// latency p50 3.50 s, p99 10.2 s against 2.97 s / 7.43 s for plain FIFO
+// service of the same requests).  Now a tick steps the requests in the order
+// of their remaining service time at a nominal rate minus their age (in
+// seconds), shortest remaining work first with aging so that a long request
+// is not passed over for ever, taking requests until their cells fill the GPU
+// (kFillCells; a 256x192x128 junction, 361k cells, runs at the full rate
+// alone, smaller cases are stepped together).  A request that has not made
+// its first flow report goes first.  On a model of that stream (rates 20 /
+// 30 G cell updates/s at FP32 / FP16, 64-step ticks) this order gives p50
+// 1.3 s and p99 4.1 s against 1.7 / 4.5 s for FIFO and 1.4 / 7.2 s for
+// processor sharing.
+constexpr double kAgeWeight = 1.0;            // seconds of remaining service per second of age
+constexpr int64_t kFillCells = 400000;        // active cells of a tick's set that fill the GPU
+constexpr double kRateF32 = 2.0e10, kRateHalf = 3.0e10;  // nominal cell updates per second
 
 }  // namespace
 
@@ -231,42 +251,85 @@ void lbm355_pool::run()
         n_building -= int64_t(fresh.size());
         lk.unlock();
 
-        std::vector<Request*> joined;
         std::vector<std::pair<Request*, int>> ended;
         for (Request* r : fresh) {
             if (r->build_rc != LBM355_OK) ended.push_back({r, r->build_rc});
-            else joined.push_back(r);
+            else running.push_back(r);
         }
-        int rc = apply(joined, {});
+        // this tick's set: the order of the tick scheduling above, up to
+        // kFillCells active cells; members of the previous set that are not
+        // in it leave their batches (they stay resident), new ones join
+        std::vector<Request*> sel;
+        if (!running.empty()) {
+            const double t_now = now_s(t0);
+            std::vector<std::pair<double, Request*>> order;
+            for (Request* r : running) {
+                double key;
+                if (r->report_every > 0 && r->st.t_first_report == 0.0) {
+                    key = -1.0e30 + r->st.t_submit;  // first report pending: first come, first served
+                } else {
+                    const double rate = lbm355::balanced_storage(r->desc.storage) ? kRateHalf : kRateF32;
+                    key = double(r->steps - r->st.steps_done) * double(r->active_cells) / rate -
+                          kAgeWeight * (t_now - r->st.t_submit);
+                }
+                order.push_back({key, r});
+            }
+            std::stable_sort(order.begin(), order.end(),
+                             [](const std::pair<double, Request*>& a, const std::pair<double, Request*>& b) {
+                                 return a.first < b.first;
+                             });
+            int64_t cells = 0;
+            for (auto& kv : order) {
+                sel.push_back(kv.second);
+                cells += kv.second->active_cells;
+                if (cells >= kFillCells) break;
+            }
+        }
+        std::vector<Request*> add, drop;
+        for (Request* r : sel)
+            if (!r->member) add.push_back(r);
+        for (Request* r : running)
+            if (r->member && std::find(sel.begin(), sel.end(), r) == sel.end()) drop.push_back(r);
+        int rc = apply(add, drop);
         if (rc != LBM355_OK) {
+            // a batch could not be created or updated (device error): the
+            // batches' memberships are then unknown, so every resident
+            // request fails with the error and the batches are rebuilt from
+            // scratch for later requests
             worker_error = lbm355_last_error();
-            for (Request* r : joined) {
+            for (auto& kv : batches) lbm355_batch_destroy(kv.second);
+            batches.clear();
+            solo.clear();
+            for (Request* r : running) {
+                r->member = false;
                 r->message = worker_error;
                 ended.push_back({r, rc});
             }
-            apply({}, joined);  // whatever did join leaves again
+            running.clear();
+            sel.clear();
         } else {
-            running.insert(running.end(), joined.begin(), joined.end());
+            for (Request* r : drop) r->member = false;
+            for (Request* r : add) r->member = true;
         }
         int64_t n = 0, updates = 0;
         double dt = 0.0;
-        if (!running.empty()) {
-            // one tick: the same step count for every running case, cut at the
-            // next completion or report point
+        if (!sel.empty()) {
+            // one tick: the same step count for every case of the set, cut at
+            // the next completion or report point of one of them
             n = tick;
-            for (Request* r : running) {
+            for (Request* r : sel) {
                 n = std::min(n, r->steps - r->st.steps_done);
                 if (r->report_every > 0) n = std::min(n, r->report_every - r->st.steps_done % r->report_every);
             }
             n = std::max<int64_t>(n, 1);
             const auto t_tick = std::chrono::steady_clock::now();
             rc = LBM355_OK;
-            // while all running cases fit the Infinity Cache together, the
+            // while the set's cases fit the Infinity Cache together, the
             // batches of a tick run concurrently on their streams; beyond that
             // they run one after another, so that each batch's cache-sized
             // groups do not evict each other
             int64_t resident = 0;
-            for (Request* r : running) resident += lbm355_case_step_bytes(r->c);
+            for (Request* r : sel) resident += lbm355_case_step_bytes(r->c);
             const bool serial = resident > lbm355::kBatchGroupBytes;
             for (auto& kv : batches) {
                 if (rc == LBM355_OK) rc = batch_enqueue(kv.second, n);
@@ -294,6 +357,10 @@ void lbm355_pool::run()
             if (rc != LBM355_OK) worker_error = lbm355_last_error();
             std::vector<Request*> still, leaving;
             for (Request* r : running) {
+                if (!r->member) {
+                    still.push_back(r);  // resident, not stepped in this tick
+                    continue;
+                }
                 if (rc != LBM355_OK) {
                     r->message = worker_error;
                     ended.push_back({r, rc});
@@ -323,6 +390,7 @@ void lbm355_pool::run()
             }
             // finished cases leave their batches before anyone can take them
             if (apply({}, leaving) != LBM355_OK && worker_error.empty()) worker_error = lbm355_last_error();
+            for (Request* r : leaving) r->member = false;
             running.swap(still);
         }
         lk.lock();
