@@ -123,3 +123,24 @@ def test_pool_refuses_bfloat16(gpu_lib):
     with pytest.raises(RuntimeError, match="bfloat16"):
         pool.submit(g.flat(), g.dims, 10, **kw)
     pool.close()
+
+
+def test_pool_steps_shortest_remaining_work_first(gpu_lib):
+    """Three requests of the same junction (about 620k active cells, more than
+    one tick's set needs to fill the GPU) submitted together with 30,000,
+    20,000 and 10,000 steps: the pool steps the one with the least remaining
+    work alone, so they finish shortest first and the shortest one finishes
+    in well under half the time the longest takes (stepped together they
+    would share every tick: 0.5 of it)."""
+    g, kw = tcpc_case(1.2, storage=_abi.F32)
+    pool = Pool(steps_per_tick=64)
+    ids = [pool.submit(g.flat(), g.dims, steps, **dict(kw, ivc_flux=kw["ivc_flux"] * (1 + 0.1 * k)))
+           for k, steps in enumerate((30000, 20000, 10000))]
+    assert pool.drain(timeout=120.0)
+    sts = [pool.poll(r) for r in ids]
+    assert all(s["state"] == _abi.REQ_DONE for s in sts)
+    assert sts[0]["active_cells"] > 400000
+    done = [s["t_done"] for s in sts]
+    assert done[2] < done[1] < done[0], done
+    assert done[2] - sts[2]["t_submit"] < 0.4 * (done[0] - sts[0]["t_submit"]), done
+    pool.close()
