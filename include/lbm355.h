@@ -332,11 +332,13 @@ int lbm355_case_autotune(lbm355_case* c, int* best);
  * No counterpart in the reference, which runs one case per process and queues
  * cases through Slurm (scripts/slurm_run_cases.py:104-198).  A pool owns one
  * worker thread that drives the current device: requests wait in FIFO order
- * until their footprint fits the device-memory budget; admitted cases of one
- * collision and storage type are stepped by one fused launch per step over
- * their concatenated brick lists, all types within the same tick; finished
- * cases leave between ticks and free their memory for waiting requests
- * (continuous batching).  Pool calls are thread-safe. */
+ * until their footprint fits the device-memory budget; admitted cases stay
+ * resident, and each tick steps the ones with the least remaining work (less
+ * their age in seconds, so that none waits for ever) until they hold enough
+ * cells to fill the GPU, those of one collision and storage type by one fused
+ * launch per step over their concatenated brick lists, all types within the
+ * same tick; finished cases leave between ticks and free their memory for
+ * waiting requests (continuous batching).  Pool calls are thread-safe. */
 typedef struct lbm355_pool lbm355_pool;
 enum {
     LBM355_REQ_QUEUED = 0,
