@@ -323,9 +323,9 @@ int lbm355_batch_set_group_bytes(lbm355_batch* b, int64_t bytes);
    *best = bricks_per_group | nontemporal << 8 | waves_per_cu << 16 |
            (group bytes >> 24) << 24 (lbm355_batch_config gives the rest) */
 int lbm355_batch_autotune(lbm355_batch* b, int* best);
-/* the same for one case without grouping; instead the 3 fastest
-   configurations are timed again, in two interleaved passes of 2 + 2W steps,
-   each keeping its faster pass (64 + 24 (2 + W) + 12 (1 + W) steps) */
+/* the same for one case without grouping; instead the 4 fastest
+   configurations are timed again, in three interleaved passes of 2 + 4W
+   steps, each keeping its fastest pass (64 + 24 (2 + W) + 12 (2 + 4W) steps) */
 int lbm355_case_autotune(lbm355_case* c, int* best);
 
 /* --- serving pool ----------------------------------------------------------

@@ -1757,7 +1757,8 @@ struct TuneConfig {
     int bpg, nt, cap;
 };
 constexpr int kTuneConfigs = 4 * 2 * 3;
-constexpr int kTuneFinalists = 3;   // configurations timed again on longer runs (single cases)
+constexpr int kTuneFinalists = 4;   // configurations timed again on longer runs (single cases)
+constexpr int kTunePasses = 3;      // interleaved passes over the finalists, each keeping its fastest
 constexpr int kTuneWarmup = 64;     // steps run before any timing (clocks settle)
 // timed window of the autotuners: a whole number of statistics groups, so
 // every window holds the same number of group flushes (a flush step costs
@@ -2264,9 +2265,11 @@ int lbm355_case_autotune(lbm355_case* c, int* best)
     if (!c) return fail(LBM355_EINVAL, "null case");
     // kTuneWarmup steps first; round 1: every configuration on 2 warm-up steps
     // and a timed window W (tune_window); round 2: the kTuneFinalists fastest,
-    // two interleaved passes of 2 + 2W steps, each keeping its faster pass
-    // (single windows of a ~20 us step are within launch jitter and clock
-    // drift of each other).
+    // kTunePasses interleaved passes of 2 + 4W steps, each keeping its fastest
+    // pass.  Single windows of a ~20 us step are within launch jitter and
+    // clock drift of each other: with 3 finalists and two passes of 2W steps
+    // the 256x192x128 junction settled on 4 bricks per workgroup and a cap of
+    // 16 in one bench run (19.45 us per step) against 18.4 us for 1 brick.
     const int W = tune_window(c->d.stats_every);
     std::vector<std::pair<double, int>> round1;
     int rc;
@@ -2284,7 +2287,7 @@ int lbm355_case_autotune(lbm355_case* c, int* best)
     std::sort(round1.begin(), round1.end());
     const int nf = std::min(kTuneFinalists, int(round1.size()));
     std::vector<double> fin(size_t(nf), 0.0);
-    for (int pass = 0; pass < 2; ++pass)
+    for (int pass = 0; pass < kTunePasses; ++pass)
         for (int r = 0; r < nf; ++r) {
             const TuneConfig t = tune_config(round1[size_t(r)].second);
             c->bricks_per_group = t.bpg;
@@ -2292,7 +2295,7 @@ int lbm355_case_autotune(lbm355_case* c, int* best)
             c->waves_cap = t.cap;
             if ((rc = lbm355_case_step(c, 2))) return rc;
             double step_ms = 0.0;
-            if ((rc = lbm355_case_time_kernel(c, 2 * W, &step_ms, nullptr))) return rc;
+            if ((rc = lbm355_case_time_kernel(c, 4 * W, &step_ms, nullptr))) return rc;
             if (pass == 0 || step_ms < fin[size_t(r)]) fin[size_t(r)] = step_ms;
         }
     int best_r = 0;

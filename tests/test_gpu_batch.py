@@ -83,7 +83,7 @@ def test_autotune_keeps_results(gpu_lib):
     for k in range(2):
         np.testing.assert_array_equal(pair[k].field("vy"), ref[k].field("vy"))
     one = Case(specs[0][0].flat(), specs[0][0].dims, **specs[0][1])
-    bpg, nt, cap = one.autotune()              # warm-up, 24 configurations, 2 passes over 3 finalists
+    bpg, nt, cap = one.autotune()              # warm-up, 24 configurations, 3 passes over 4 finalists
     assert bpg in (1, 2, 4, 8) and nt in (0, 1) and cap in (0, 16, 12)
     m = one.info()["steps"]
     assert m > 24 * 10
