@@ -163,6 +163,12 @@ struct Pop<S_F16> {
         const float h = float(__builtin_bit_cast(_Float16, ld_u16<A>(r, i * 2u)));
         return fmaf(h, 1.0f / float(1 << half_scale_exp(q)), wgt<float>(q));
     }
+    // the stored deviation f_q - w_q itself (exact in FP32)
+    template <int A = 0>
+    __device__ static R ld_dev(rsrc_t r, uint32_t i, int q)
+    {
+        return float(__builtin_bit_cast(_Float16, ld_u16<A>(r, i * 2u))) * (1.0f / float(1 << half_scale_exp(q)));
+    }
     // x * 2^-112 moves the FP32 exponent field onto the IEEE-half one (bias
     // 127 -> 15; FP32 denormals are kept, which covers the half subnormals),
     // so bits 13..27 of its FP32 image are the half magnitude; adding 13
@@ -171,7 +177,13 @@ struct Pop<S_F16> {
     template <int A = 0>
     __device__ static void st(rsrc_t r, uint32_t i, int q, R v, uint32_t rnd)
     {
-        const float x = (v - wgt<float>(q)) * float(1 << half_scale_exp(q));
+        st_dev<A>(r, i, q, v - wgt<float>(q), rnd);
+    }
+    // stores the deviation d = f_q - w_q
+    template <int A = 0>
+    __device__ static void st_dev(rsrc_t r, uint32_t i, int q, R d, uint32_t rnd)
+    {
+        const float x = d * float(1 << half_scale_exp(q));
         const uint32_t b = __float_as_uint(fabsf(x) * 0x1p-112f) + (rnd & 0x1FFFu);
         const uint32_t h = (b >> 13) | ((__float_as_uint(x) >> 16) & 0x8000u);
         st_u16<A>(r, i * 2u, uint16_t(h));
@@ -200,9 +212,19 @@ struct Pop<S_BF16> {
         return __uint_as_float(uint32_t(ld_u16<A>(r, i * 2u)) << 16) + wgt<float>(q);
     }
     template <int A = 0>
+    __device__ static R ld_dev(rsrc_t r, uint32_t i, int)
+    {
+        return __uint_as_float(uint32_t(ld_u16<A>(r, i * 2u)) << 16);
+    }
+    template <int A = 0>
     __device__ static void st(rsrc_t r, uint32_t i, int q, R v, uint32_t rnd)
     {
-        st_u16<A>(r, i * 2u, uint16_t((__float_as_uint(v - wgt<float>(q)) + (rnd & 0xFFFFu)) >> 16));
+        st_dev<A>(r, i, q, v - wgt<float>(q), rnd);
+    }
+    template <int A = 0>
+    __device__ static void st_dev(rsrc_t r, uint32_t i, int, R d, uint32_t rnd)
+    {
+        st_u16<A>(r, i * 2u, uint16_t((__float_as_uint(d) + (rnd & 0xFFFFu)) >> 16));
     }
     __device__ static void put(T* p, int64_t i, int q, R v) { p[i] = to_bf16(v - wgt<float>(q)); }
 };
@@ -273,6 +295,22 @@ __device__ __forceinline__ void from_central(R& a, R& b, R& c, R u)
     c = fp;
 }
 
+// the same step on deviations from the rest state: the column's k0 is
+// K0 + a, its k2 K0/3 + c, its output (K0/6, 2 K0/3, K0/6) + (a, b, c).  K0
+// enters only through the velocity terms, and b = a - fm - fp keeps the
+// column's mass deviation exactly
+template <class R>
+__device__ __forceinline__ void from_central_dev(R& a, R& b, R& c, R u, R K0)
+{
+    const R k0 = a, k1 = b, k2 = c;
+    const R k0f = k0 + K0;
+    const R fm = R(0.5) * (u * u - u) * k0f + (u - R(0.5)) * k1 + R(0.5) * k2;
+    const R fp = fm + u * k0f + k1;
+    a = fm;
+    b = k0 - fm - fp;
+    c = fp;
+}
+
 // Moments of a population set up to second order: density, raw momentum
 // J = sum c f and raw second moments P_ab = sum c_a c_b f, summed over the 13
 // opposite pairs (q, 26-q).  The integer lattice coefficients are resolved at
@@ -327,8 +365,16 @@ __device__ __forceinline__ constexpr int K(int a, int b, int c) { return 9 * a +
 // kappa_ab = P_ab - u_a J_b - u_b J_a + rho u_a u_b, which replaces the
 // forward chimera transform; the backward chimera (lbm_col_cum.h:319-415)
 // rebuilds the 27 populations.
-template <class R>
-__device__ __forceinline__ void collide_cumulant(R (&m)[27], const Moments<R>& M, R ux, R uy, R uz, R om)
+// With DEV the populations are the deviations f_q - w_q from the rest state
+// (the two-byte storage types keep exactly those): M then holds the full rho
+// but the raw second moments of the deviations (P_aa less 1/3), dr = rho - 1,
+// and the diagonal 2nd, 4th and 6th-order central moments are carried as
+// their deviations from 1/3, 1/9 and 1/27.  This is the same collision with
+// no quantity near 1 rounded in FP32 on the way: a post-collision population
+// of a cell at rest is w_q = 8/27 ... 1/216 + a small part, and rounding the
+// whole value each step is a systematic error in the conserved moments
+template <class R, bool DEV = false>
+__device__ __forceinline__ void collide_cumulant(R (&m)[27], const Moments<R>& M, R dr, R ux, R uy, R uz, R om)
 {
     const R rho = M.rho;
     const R inv = Fm<R>::rcp(rho);
@@ -344,32 +390,52 @@ __device__ __forceinline__ void collide_cumulant(R (&m)[27], const Moments<R>& M
     const R dxy = om1 * (k200 - k020);
     const R dxz = om1 * (k200 - k002);
     const R third = R(1) / R(3);
-    const R c200 = (dxy + dxz + rho) * third;
-    const R c020 = (dxz - R(2) * dxy + rho) * third;
-    const R c002 = (dxy - R(2) * dxz + rho) * third;
-    const R k211 = (c200 * c011 + R(2) * c101 * c110) * inv;
-    const R k121 = (c020 * c101 + R(2) * c110 * c011) * inv;
-    const R k112 = (c002 * c110 + R(2) * c011 * c101) * inv;
-    const R k220 = (c020 * c200 + R(2) * c110 * c110) * inv;
-    const R k022 = (c002 * c020 + R(2) * c011 * c011) * inv;
-    const R k202 = (c200 * c002 + R(2) * c101 * c101) * inv;
-    const R k222 = (c200 * k022 + c020 * k202 + c002 * k220 + R(4) * (c011 * k211 + c101 * k121 + c110 * k112)) * inv -
-                   (R(16) * c110 * c101 * c011 + R(4) * (c101 * c101 * c020 + c011 * c011 * c200 + c110 * c110 * c002) +
-                    R(2) * c200 * c020 * c002) * (inv * inv);
+    const R r0 = DEV ? dr : rho;
+    const R c200 = (dxy + dxz + r0) * third;
+    const R c020 = (dxz - R(2) * dxy + r0) * third;
+    const R c002 = (dxy - R(2) * dxz + r0) * third;
+    const R C200 = DEV ? c200 + third : c200, C020 = DEV ? c020 + third : c020, C002 = DEV ? c002 + third : c002;
+    const R k211 = (C200 * c011 + R(2) * c101 * c110) * inv;
+    const R k121 = (C020 * c101 + R(2) * c110 * c011) * inv;
+    const R k112 = (C002 * c110 + R(2) * c011 * c101) * inv;
+    R k220, k022, k202, k222;
+    if constexpr (DEV) {
+        // k_220 - 1/9 = ((c020 + c200)/3 + c020 c200 + 2 c110^2 - dr/9) / rho
+        // with c_aa the deviations; k_222 - 1/27 likewise, its constant parts
+        // summing to dr (1 - dr) / 27 / rho^2 (c200 + c020 + c002 = dr)
+        const R n9 = dr * (R(1) / R(9));
+        k220 = ((c020 + c200) * third + c020 * c200 + R(2) * c110 * c110 - n9) * inv;
+        k022 = ((c002 + c020) * third + c002 * c020 + R(2) * c011 * c011 - n9) * inv;
+        k202 = ((c200 + c002) * third + c200 * c002 + R(2) * c101 * c101 - n9) * inv;
+        const R s1 = n9 + (k022 + k202 + k220) * third + (c200 * k022 + c020 * k202 + c002 * k220) +
+                     R(4) * (c011 * k211 + c101 * k121 + c110 * k112);
+        const R s3 = R(2) * n9 + R(2) * third * (c200 * c020 + c020 * c002 + c002 * c200) + R(2) * c200 * c020 * c002 +
+                     R(16) * c110 * c101 * c011 + R(4) * (c101 * c101 * C020 + c011 * c011 * C200 + c110 * c110 * C002);
+        k222 = s1 * inv + (dr * (R(1) - dr) * (R(1) / R(27)) - s3) * (inv * inv);
+    } else {
+        k220 = (c020 * c200 + R(2) * c110 * c110) * inv;
+        k022 = (c002 * c020 + R(2) * c011 * c011) * inv;
+        k202 = (c200 * c002 + R(2) * c101 * c101) * inv;
+        k222 = (c200 * k022 + c020 * k202 + c002 * k220 + R(4) * (c011 * k211 + c101 * k121 + c110 * k112)) * inv -
+               (R(16) * c110 * c101 * c011 + R(4) * (c101 * c101 * c020 + c011 * c011 * c200 + c110 * c110 * c002) +
+                R(2) * c200 * c020 * c002) * (inv * inv);
+    }
 
     // backward along x for the 9 (y,z)-moment columns; the 3rd and 5th order
-    // central moments are zero, which the per-column code below drops
+    // central moments are zero, which the per-column code below drops.  k0f
+    // is the column's full zeroth moment, k0 the one whose mass f- + f0 + f+
+    // takes (the same but for the rest part under DEV)
     const R a0 = R(0.5) * (ux * ux - ux), a1 = ux - R(0.5);  // f- = a0 k0 + a1 k1 + k2/2
-    auto bx = [&](int col, R k0, R k1, R k2) {
-        const R fm = a0 * k0 + a1 * k1 + R(0.5) * k2;
-        const R fp = fm + ux * k0 + k1;
+    auto bx = [&](int col, R k0f, R k0, R k1, R k2) {
+        const R fm = a0 * k0f + a1 * k1 + R(0.5) * k2;
+        const R fp = fm + ux * k0f + k1;
         m[col] = fm;
         m[9 + col] = k0 - fm - fp;
         m[18 + col] = fp;
     };
-    auto bx0 = [&](int col, R k0, R k2) {  // k1 = 0
-        const R fm = a0 * k0 + R(0.5) * k2;
-        const R fp = fm + ux * k0;
+    auto bx0 = [&](int col, R k0f, R k0, R k2) {  // k1 = 0
+        const R fm = a0 * k0f + R(0.5) * k2;
+        const R fp = fm + ux * k0f;
         m[col] = fm;
         m[9 + col] = k0 - fm - fp;
         m[18 + col] = fp;
@@ -382,22 +448,32 @@ __device__ __forceinline__ void collide_cumulant(R (&m)[27], const Moments<R>& M
         m[18 + col] = fp;
     };
     // columns indexed by (b,c) = y,z orders: col = 3b + c
-    bx(K(0, 0, 0), rho, -k100, c200);
-    bx(K(0, 1, 0), -k010, c110, R(0));
-    bx0(K(0, 2, 0), c020, k220);
-    bx(K(0, 0, 1), -k001, c101, R(0));
-    bx0(K(0, 1, 1), c011, k211);
+    bx(K(0, 0, 0), rho, r0, -k100, c200);
+    bx(K(0, 1, 0), -k010, -k010, c110, R(0));
+    bx0(K(0, 2, 0), C020, c020, k220);
+    bx(K(0, 0, 1), -k001, -k001, c101, R(0));
+    bx0(K(0, 1, 1), c011, c011, k211);
     bx1(K(0, 2, 1), k121);
-    bx0(K(0, 0, 2), c002, k202);
+    bx0(K(0, 0, 2), C002, c002, k202);
     bx1(K(0, 1, 2), k112);
-    bx0(K(0, 2, 2), k022, k222);
-    // then along y and z (general triples)
+    bx0(K(0, 2, 2), DEV ? k022 + R(1) / R(9) : k022, k022, k222);
+    // then along y and z (general triples).  At rest the x pass leaves
+    // w1(a) Z(c) in the y column (a, c), w1 = (1/6, 2/3, 1/6) the 1-D weights
+    // and Z = (1, 0, 1/3) the 1-D rest moments, and the y pass w1(a) w1(b) in
+    // the z column (a, b)
+    constexpr double w1[3] = {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0}, z1[3] = {1.0, 0.0, 1.0 / 3.0};
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) from_central(m[9 * a + c], m[9 * a + 3 + c], m[9 * a + 6 + c], uy);
+        for (int c = 0; c < 3; ++c) {
+            if (DEV && c != 1) from_central_dev(m[9 * a + c], m[9 * a + 3 + c], m[9 * a + 6 + c], uy, R(w1[a] * z1[c]));
+            else from_central(m[9 * a + c], m[9 * a + 3 + c], m[9 * a + 6 + c], uy);
+        }
 #pragma unroll
-    for (int a = 0; a < 9; ++a) from_central(m[3 * a], m[3 * a + 1], m[3 * a + 2], uz);
+    for (int a = 0; a < 9; ++a) {
+        if (DEV) from_central_dev(m[3 * a], m[3 * a + 1], m[3 * a + 2], uz, R(w1[a / 3] * w1[a % 3]));
+        else from_central(m[3 * a], m[3 * a + 1], m[3 * a + 2], uz);
+    }
 }
 
 // 1-D factors of the product-form equilibrium: phi(0) = 2/3 - u^2,
@@ -718,11 +794,12 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) { return (uint64_t(1) 
 // 13 populations q > 13 one cell downstream, instead of the odd step shifting
 // all 26 (512x384x256 FP16: 93.7 -> 84.1 us per step).  The loads of a
 // step: even steps read slot q, odd steps slot 26 - q
-template <int S>
+template <int S, bool DEV = false>
 __device__ __forceinline__ void balanced_loads(const rsrc_t& F, const int (&t)[27], uint32_t own, int i, int j, int k,
                                                int lane, int parity, typename Pop<S>::R (&f)[27])
 {
     using P = Pop<S>;
+    auto ld = [&](uint32_t e, int q) { if constexpr (DEV) return P::ld_dev(F, e, q); else return P::ld(F, e, q); };
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
         const uint32_t sl = uint32_t(parity == 0 ? q : 26 - q) * kLanes;
@@ -731,9 +808,9 @@ __device__ __forceinline__ void balanced_loads(const rsrc_t& F, const int (&t)[2
             const int si = i - cx, sj = j - cy, sk = k - cz;
             const int nb = pick_brick(t, -cx, -cy, -cz, (si & ~(kBX - 1)) != 0, (sj & ~(kBY - 1)) != 0,
                                       (sk & ~(kBZ - 1)) != 0);
-            f[q] = P::ld(F, uint32_t(nb + brick_lane(si, sj, sk)) + sl, q);
+            f[q] = ld(uint32_t(nb + brick_lane(si, sj, sk)) + sl, q);
         } else {
-            f[q] = P::ld(F, own + sl, q);
+            f[q] = ld(own + sl, q);
         }
     }
 }
@@ -787,11 +864,14 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
     const int i = lane & (kBX - 1), j = (lane >> kSX) & (kBY - 1), k = lane >> (kSX + kSY);
     const uint32_t own = uint32_t(t[13]) + uint32_t(lane);
 
+    // the cumulant on two-byte storage runs on the stored deviations
+    // f_q - w_q from load to store (collide_cumulant<R, true>)
+    constexpr bool kDev = kBalanced<S> && MODEL == M_CUM;
     R f[27];
     R rho = R(1), ux = R(0), uy = R(0), uz = R(0);
     if (fluid) {
         if constexpr (kBalanced<S>) {
-            balanced_loads<S>(F, t, own, i, j, k, lane, parity, f);
+            balanced_loads<S, kDev>(F, t, own, i, j, k, lane, parity, f);
         } else if (parity == 0) {
 #pragma unroll
             for (int q = 0; q < 27; ++q) f[q] = P::template ld<kEvenAux>(F, own + uint32_t(q * kLanes), q);
@@ -814,7 +894,11 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
             Fz = fo[(b * 3 + 2) * kLanes + lane];
         }
         // density and momentum (lbm_common.h:16-39) with the second moments
-        const Moments<R> M = moments2<R>(f);
+        // (with kDev: the sums of the deviations, so M.rho is rho - 1 and
+        // M.p_aa is P_aa - 1/3; the other moments of w vanish)
+        Moments<R> M = moments2<R>(f);
+        const R dr = kDev ? M.rho : M.rho - R(1);
+        if constexpr (kDev) M.rho += R(1);
         rho = M.rho;
         const R ir = Fm<R>::rcp(rho);
         ux = (M.jx + R(0.5) * Fx) * ir;
@@ -823,7 +907,7 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
         R nu = CyK<R>::nu_ref(C);
         if (C.cy_mode == 1) nu = neq_viscosity<R>(M, ux, uy, uz, C);
         const R om = Fm<R>::rcp(R(3) * nu + R(0.5));
-        if constexpr (MODEL == M_CUM) collide_cumulant<R>(f, M, ux, uy, uz, om);
+        if constexpr (MODEL == M_CUM) collide_cumulant<R, kDev>(f, M, dr, ux, uy, uz, om);
         else if constexpr (MODEL == M_BGK) collide_bgk<R>(f, rho, ux, uy, uz, Fx, Fy, Fz, om);
         else if constexpr (MODEL == M_MRT) collide_mrt<R>(f, M, ux, uy, uz, nu);
         else collide_kbc<R>(f, M, ux, uy, uz, Fx, Fy, Fz, nu);
@@ -848,6 +932,10 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
             rho = R(1);
         }
         feq_boundary<R>(f, rho, ux, uy, uz);
+        if constexpr (kDev) {
+#pragma unroll
+            for (int q = 0; q < 27; ++q) f[q] -= wgt<R>(q);
+        }
     }
     if (fluid || bc) {
         // random bits of the two-byte stochastic rounding (unused at FP32/FP64)
@@ -865,9 +953,12 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
                     const int di = i + cx, dj = j + cy, dk = k + cz;
                     const int nb = pick_brick(t, cx, cy, cz, (di & ~(kBX - 1)) != 0, (dj & ~(kBY - 1)) != 0,
                                               (dk & ~(kBZ - 1)) != 0);
-                    P::st(F, uint32_t(nb + brick_lane(di, dj, dk)) + sl, q, f[q], rnd_q(H, q));
+                    const uint32_t e = uint32_t(nb + brick_lane(di, dj, dk)) + sl;
+                    if constexpr (kDev) P::st_dev(F, e, q, f[q], rnd_q(H, q));
+                    else P::st(F, e, q, f[q], rnd_q(H, q));
                 } else {
-                    P::st(F, own + sl, q, f[q], rnd_q(H, q));
+                    if constexpr (kDev) P::st_dev(F, own + sl, q, f[q], rnd_q(H, q));
+                    else P::st(F, own + sl, q, f[q], rnd_q(H, q));
                 }
             }
         } else if (parity == 0) {

@@ -10,9 +10,9 @@ output path (FD strain, core.h:407-576) and the LPA share of the mean outlet
 flow (compute_flows.py:141-147 on the label-5/6 planes).
 
 Measured on the MI355X (velocity / rho - 1 / shear rate / split points):
-FP32 2.2e-6 / 2.4e-6 / 2.6e-6 / 0.00000; FP16-offset 2.4e-4 / 1.4e-4 / 3.0e-4 /
-0.00007; BF16-offset 1.9e-3 / 1.1e-3 / 2.4e-3 / 0.0013.  Tolerances: FP32
-1e-4 on each field; FP16-offset 1e-3 on velocity and
+FP32 2.2e-6 / 2.2e-6 / 2.7e-6 / 0.00000; FP16-offset 2.3e-4 / 1.4e-4 / 3.0e-4 /
+0.00010; BF16-offset 1.8e-3 / 1.1e-3 / 2.4e-3 / 0.0042.  Tolerances: FP32
+1e-4 on each field; FP16-offset 1e-3 (the accuracy bar) on velocity and
 rho - 1, 2e-3 on the shear rate; BF16-offset 5e-3 / 5e-3 / 1e-2 (it is
 refused for serving); the flow split within 0.5 share points for every
 type.
