@@ -169,24 +169,22 @@ struct Pop<S_F16> {
     {
         return float(__builtin_bit_cast(_Float16, ld_u16<A>(r, i * 2u))) * (1.0f / float(1 << half_scale_exp(q)));
     }
-    // x * 2^-112 moves the FP32 exponent field onto the IEEE-half one (bias
-    // 127 -> 15; FP32 denormals are kept, which covers the half subnormals),
-    // so bits 13..27 of its FP32 image are the half magnitude; adding 13
-    // random bits first rounds it stochastically.  |x| < 2 in practice
-    // (x = (f/w - 1) 16/27); no clamp is needed below the half range (65504).
     template <int A = 0>
     __device__ static void st(rsrc_t r, uint32_t i, int q, R v, uint32_t rnd)
     {
         st_dev<A>(r, i, q, v - wgt<float>(q), rnd);
     }
-    // stores the deviation d = f_q - w_q
+    // stores the deviation d = f_q - w_q, scaled, rounded stochastically by
+    // gfx950's v_cvt_sr_f16_f32 with the word rnd (half subnormals included,
+    // FP32 denormals being kept).  |x| < 2 in practice (x = (f/w - 1) 16/27),
+    // far below the half range (65504)
     template <int A = 0>
     __device__ static void st_dev(rsrc_t r, uint32_t i, int q, R d, uint32_t rnd)
     {
+        typedef __fp16 half2_t __attribute__((ext_vector_type(2)));
         const float x = d * float(1 << half_scale_exp(q));
-        const uint32_t b = __float_as_uint(fabsf(x) * 0x1p-112f) + (rnd & 0x1FFFu);
-        const uint32_t h = (b >> 13) | ((__float_as_uint(x) >> 16) & 0x8000u);
-        st_u16<A>(r, i * 2u, uint16_t(h));
+        const half2_t h = __builtin_amdgcn_cvt_sr_f16_f32(half2_t{0, 0}, x, rnd, false);
+        st_u16<A>(r, i * 2u, uint16_t(__builtin_bit_cast(uint32_t, h) & 0xFFFFu));
     }
     __device__ static void put(T* p, int64_t i, int q, R v)
     {
@@ -349,6 +347,52 @@ __device__ __forceinline__ Moments<R> moments2(const R (&f)[27])
     }
     m.rho = rho; m.jx = jx; m.jy = jy; m.jz = jz;
     m.pxx = pxx; m.pyy = pyy; m.pzz = pzz; m.pxy = pxy; m.pxz = pxz; m.pyz = pyz;
+    return m;
+}
+
+// The same moments summed one axis at a time, as the chimera transform does:
+// per (cy, cz) column the x sums f+ + f- and differences f+ - f-, then per cz
+// the y sums and differences of those, then the z ones; 68 additions instead
+// of 111.  Over full populations (values near w_q) this order rounds worse
+// than the opposite-pair differences; over the deviations f - w of the
+// two-byte storage types the sums are small and it costs no accuracy
+template <class R>
+__device__ __forceinline__ Moments<R> moments2_axis(const R (&f)[27])
+{
+    // x: column (b, c) = (cy + 1, cz + 1) holds q = 3b + c + 9a, a = cx + 1
+    R m0[9], m1[9], m2[9];  // sum f, sum cx f, sum cx^2 f over the column
+#pragma unroll
+    for (int col = 0; col < 9; ++col) {
+        const R fm = f[col], f0 = f[9 + col], fp = f[18 + col];
+        m2[col] = fp + fm;
+        m1[col] = fp - fm;
+        m0[col] = m2[col] + f0;
+    }
+    // y, per cz column c: sums over cy of m0, cy m0, cy^2 m0, m1, cy m1, m2
+    R t0[3], b0[3], a0[3], t1[3], b1[3], t2[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        a0[c] = m0[6 + c] + m0[c];
+        b0[c] = m0[6 + c] - m0[c];
+        t0[c] = a0[c] + m0[3 + c];
+        const R s1 = m1[6 + c] + m1[c];
+        b1[c] = m1[6 + c] - m1[c];
+        t1[c] = s1 + m1[3 + c];
+        t2[c] = (m2[6 + c] + m2[c]) + m2[3 + c];
+    }
+    // z
+    Moments<R> m;
+    const R pz = t0[2] + t0[0];
+    m.rho = pz + t0[1];
+    m.jz = t0[2] - t0[0];
+    m.pzz = pz;
+    m.jy = (b0[2] + b0[0]) + b0[1];
+    m.pyz = b0[2] - b0[0];
+    m.pyy = (a0[2] + a0[0]) + a0[1];
+    m.jx = (t1[2] + t1[0]) + t1[1];
+    m.pxz = t1[2] - t1[0];
+    m.pxy = (b1[2] + b1[0]) + b1[1];
+    m.pxx = (t2[2] + t2[0]) + t2[1];
     return m;
 }
 
@@ -896,7 +940,9 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
         // density and momentum (lbm_common.h:16-39) with the second moments
         // (with kDev: the sums of the deviations, so M.rho is rho - 1 and
         // M.p_aa is P_aa - 1/3; the other moments of w vanish)
-        Moments<R> M = moments2<R>(f);
+        Moments<R> M;
+        if constexpr (kDev) M = moments2_axis<R>(f);
+        else M = moments2<R>(f);
         const R dr = kDev ? M.rho : M.rho - R(1);
         if constexpr (kDev) M.rho += R(1);
         rho = M.rho;
