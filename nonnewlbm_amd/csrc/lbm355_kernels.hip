@@ -521,14 +521,20 @@ __device__ __forceinline__ void collide_cumulant(R (&m)[27], const Moments<R>& M
 }
 
 // 1-D factors of the product-form equilibrium: phi(0) = 2/3 - u^2,
-// phi(+-1) = (u^2 +- u + 1/3)/2 (lbm_col_bgk.h:43-53, lbm_eq_inv_cum.h:19-49)
+// phi(+-1) = (u^2 +- u + 1/3)/2 (lbm_col_bgk.h:43-53, lbm_eq_inv_cum.h:19-49).
+// The three sum to 1, so that sum_q f_eq = rho.  The rounded 2/3 and 1/3
+// must keep that: in FP32 both round up, their sum is 1 + 3e-8, and with it
+// every BGK or KBC step added 9e-8 rho to each cell (40,000-step junction:
+// velocity 5e-3 off FP64).  1/3 is therefore taken as 1 - (2/3 rounded),
+// which is exact (Sterbenz) and makes the pair sum to exactly 1
 template <class R>
 __device__ __forceinline__ void phi3(R u, R (&p)[3])
 {
+    constexpr R k23 = R(2) / R(3), k13 = R(1) - k23;
     const R uu = u * u;
-    p[1] = R(2) / R(3) - uu;
-    p[0] = R(0.5) * (uu - u + R(1) / R(3));
-    p[2] = R(0.5) * (uu + u + R(1) / R(3));
+    p[1] = k23 - uu;
+    p[0] = R(0.5) * (uu - u + k13);
+    p[2] = R(0.5) * (uu + u + k13);
 }
 
 // boundary equilibrium f_eq^INV_CUM(rho, u) including the extra rho of feq_pmm
