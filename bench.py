@@ -350,21 +350,24 @@ def models(steps: int) -> dict:
     return out
 
 
-def precision_study(scale: float, steps: int) -> dict:
-    """Storage precision against FP64 on the pulsatile junction: cumulant +
-    CY-A (neq), parabolic inflows 0.04/0.03 with u(t) = u (1 + 0.3 sin(2 pi t /
-    4000)), FP64 statistics.  Relative L2 over the non-wall cells of u and of
-    the output-path gamma_dot and wss_proxy (the VTK formulas, sim_tcpc.cu:
-    697-714), and the LPA share of the mean outlet flow."""
+def precision_study(scale: float, steps: int, collision: int | None = None, storages=None) -> dict:
+    """Storage precision against FP64 on the pulsatile junction: cumulant (or
+    ``collision``) + CY-A (neq), parabolic inflows 0.04/0.03 with u(t) = u (1 +
+    0.3 sin(2 pi t / 4000)), FP64 statistics.  Relative L2 over the non-wall
+    cells of u and of the output-path gamma_dot and wss_proxy (the VTK
+    formulas, sim_tcpc.cu:697-714), and the LPA share of the mean outlet
+    flow."""
     from nonnewlbm_amd import _abi
     from nonnewlbm_amd.lbm355 import Case
     from nonnewlbm_amd.materials import material_by_name
     mat = material_by_name("cy-a")
-    res: dict = {"workload": f"TCPC junction x{scale} pulsatile (A=0.3, T=4000 steps), CUM + CY-A neq, {steps} steps",
-                 "storages": {}}
+    model = _abi.CUM if collision is None else collision
+    name = {v: k for k, v in _abi.COLLISIONS.items() if k in ("cumulant", "bgk", "mrt_les", "kbc_n4")}[model]
+    res: dict = {"workload": f"TCPC junction x{scale} pulsatile (A=0.3, T=4000 steps), {name} + CY-A neq, "
+                             f"{steps} steps", "storages": {}}
     ref = None
-    for st in (_abi.F64, _abi.F32, _abi.F16_OFFSET, _abi.BF16_OFFSET):
-        g, kw = make_case(scale, st, _abi.CUM, 0.01)
+    for st in storages or (_abi.F64, _abi.F32, _abi.F16_OFFSET, _abi.BF16_OFFSET):
+        g, kw = make_case(scale, st, model, 0.01)
         kw.update(pulse_amp=0.3, pulse_period=4000.0)
         c = Case(g.flat(), g.dims, **kw)
         t0 = time.perf_counter()
@@ -401,7 +404,7 @@ def precision_study(scale: float, steps: int) -> dict:
             row.update(u_rel_l2=rl2(u, ref[0]), rho_minus_1_rel_l2=rl2(rho1, ref[4]), gamma_dot_rel_l2=rl2(gdot, ref[1]),
                        wss_rel_l2=rl2(wss, ref[2]), split_abs_diff_percent=round(100 * abs(split - ref[3]), 5))
         res["storages"][_abi.STORAGE_NAMES[st]] = row
-        print(f"# precision {_abi.STORAGE_NAMES[st]}: {row}", file=sys.stderr, flush=True)
+        print(f"# precision {name} {_abi.STORAGE_NAMES[st]}: {row}", file=sys.stderr, flush=True)
     return res
 
 
@@ -548,6 +551,8 @@ def main() -> int:
     ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
                     help="also run the FP64/FP32/FP16/BF16 storage study for STEPS steps")
     ap.add_argument("--precision-scale", type=float, default=2.0)
+    ap.add_argument("--precision-models", type=int, default=0, metavar="STEPS",
+                    help="the same study for BGK, MRT_LES and KBC_N4 (FP64 / FP32 / FP16-offset)")
     ap.add_argument("--launch", default="auto", metavar="BPG,NT,CAP",
                     help="launch configuration of the 256x192x128 line (bricks per workgroup, non-temporal even "
                          "step, wavefronts per CU); default: autotune")
@@ -715,6 +720,11 @@ def main() -> int:
 
     if args.precision_study > 0:
         out["precision_study"] = precision_study(args.precision_scale, args.precision_study)
+    if args.precision_models > 0:
+        from nonnewlbm_amd import _abi
+        out["precision_models"] = {
+            name: precision_study(args.precision_scale, args.precision_models, m, (_abi.F64, _abi.F32, _abi.F16_OFFSET))
+            for name, m in (("bgk", _abi.BGK), ("mrt_les", _abi.MRT), ("kbc_n4", _abi.KBC))}
 
     if not args.no_channel:
         out["channel"] = channel_line()

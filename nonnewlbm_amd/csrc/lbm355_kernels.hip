@@ -560,37 +560,97 @@ __device__ __forceinline__ R dot_c(int q, R Fx, R Fy, R Fz)
     return v;
 }
 
-// improved BGK with product-form equilibrium and source term
-// (lbm_col_bgk.h:14-140)
+// The product-form equilibrium as deviations from the rest state, from small
+// quantities only: phi_a = p0_a (1 + e_a) with p0 = (1/6, 2/3, 1/6) the 1-D
+// rest weights, e(0) = -3/2 u^2 and e(+-1) = 3 (u^2 +- u), so that
+// f_eq,q - w_q = w_q (rho - 1 + rho E_q), E_q = prod_a (1 + e_a) - 1.  The
+// 1-D corrections p0_a e_a sum to zero, so sum_q w_q E_q = 0 and the mass of
+// the deviations is rho - 1 up to rounding of small numbers
 template <class R>
-__device__ __forceinline__ void collide_bgk(R (&f)[27], R rho, R ux, R uy, R uz, R Fx, R Fy, R Fz, R om)
+__device__ __forceinline__ void eq_dev3(R u, R (&e)[3])
 {
-    R px[3], py[3], pz[3];
-    phi3(ux, px); phi3(uy, py); phi3(uz, pz);
+    const R uu = u * u;
+    e[1] = R(-1.5) * uu;
+    e[0] = R(3) * (uu - u);
+    e[2] = R(3) * (uu + u);
+}
+// With e_xy = (1 + e_x)(1 + e_y) - 1 per (x, y) pair, rho - 1 + rho E_q =
+// A + B e_z with A = rho - 1 + rho e_xy and B = rho (1 + e_xy): two
+// operations per direction once the nine pairs are formed
+template <class R>
+struct EqDev {
+    R A[9], B[9], ez[3];
+    __device__ EqDev(R rho, R dr, R ux, R uy, R uz)
+    {
+        R ex[3], ey[3];
+        eq_dev3(ux, ex); eq_dev3(uy, ey); eq_dev3(uz, ez);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const R exy = ex[a] + ey[b] + ex[a] * ey[b];
+                A[3 * a + b] = dr + rho * exy;
+                B[3 * a + b] = rho + rho * exy;
+            }
+    }
+    // f_eq,q - w_q
+    __device__ R operator()(int q) const { return wgt<R>(q) * (A[q / 3] + B[q / 3] * ez[q % 3]); }
+};
+
+// improved BGK with product-form equilibrium and source term
+// (lbm_col_bgk.h:14-140).  DEV: f holds f - w (in and out), dr = rho - 1
+template <class R, bool DEV = false>
+__device__ __forceinline__ void collide_bgk(R (&f)[27], R rho, R dr, R ux, R uy, R uz, R Fx, R Fy, R Fz, R om)
+{
     const R src = (R(1) - R(0.5) * om) * R(3) * Fm<R>::rcp(rho);
     const R uF = ux * Fx + uy * Fy + uz * Fz;
+    if constexpr (DEV) {
+        const EqDev<R> E(rho, dr, ux, uy, uz);
 #pragma unroll
-    for (int q = 0; q < 27; ++q) {
-        const R feq = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
-        f[q] += (feq - f[q]) * om + src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
+        for (int q = 0; q < 27; ++q) {
+            const R de = E(q);
+            f[q] += (de - f[q]) * om + src * (dot_c(q, Fx, Fy, Fz) - uF) * (de + wgt<R>(q));
+        }
+    } else {
+        R px[3], py[3], pz[3];
+        phi3(ux, px); phi3(uy, py); phi3(uz, pz);
+#pragma unroll
+        for (int q = 0; q < 27; ++q) {
+            const R feq = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
+            f[q] += (feq - f[q]) * om + src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
+        }
     }
 }
 
 // regularised collision with a Smagorinsky rate, "MRT_LES"
 // (lbm_col_mrt.h:14-77); the force is not applied, as in the reference
-template <class R>
-__device__ __forceinline__ void collide_mrt(R (&f)[27], const Moments<R>& M, R ux, R uy, R uz, R nu)
+// DEV: f holds f - w (in and out) and M the moments of the deviations (rho
+// full, P_aa less 1/3), dr = rho - 1.  With P~ = P - rho/3 I the relaxed
+// moments less their isotropic part, f_q - w_q = w_q (dr + 3 rho c.u +
+// 9/2 c.P~.c - 3/2 tr P~): the same populations, from small numbers only
+template <class R, bool DEV = false>
+__device__ __forceinline__ void collide_mrt(R (&f)[27], const Moments<R>& M, R dr, R ux, R uy, R uz, R nu)
 {
     const R rho = M.rho;
     R Pxx = M.pxx, Pyy = M.pyy, Pzz = M.pzz, Pxy = M.pxy, Pxz = M.pxz, Pyz = M.pyz;
     const R third = R(1) / R(3);
-    const R nxx = Pxx - rho * (third + ux * ux), nyy = Pyy - rho * (third + uy * uy), nzz = Pzz - rho * (third + uz * uz);
+    R nxx, nyy, nzz;
+    if constexpr (DEV) {
+        const R r3 = dr * third;
+        nxx = Pxx - r3 - rho * ux * ux; nyy = Pyy - r3 - rho * uy * uy; nzz = Pzz - r3 - rho * uz * uz;
+    } else {
+        nxx = Pxx - rho * (third + ux * ux); nyy = Pyy - rho * (third + uy * uy); nzz = Pzz - rho * (third + uz * uz);
+    }
     const R nxy = Pxy - rho * ux * uy, nxz = Pxz - rho * ux * uz, nyz = Pyz - rho * uy * uz;
     const R Q = R(2) * (nxx * nxx + nyy * nyy + nzz * nzz + R(2) * (nxy * nxy + nxz * nxz + nyz * nyz));
     const R tau = R(3) * nu + R(0.5);
     const R om = Fm<R>::div(R(2), Fm<R>::sqrt_(tau * tau + R(18) * R(0.0342) * Fm<R>::sqrt_(Q) * Fm<R>::rcp(rho)) + tau);
     Pxx -= om * nxx; Pyy -= om * nyy; Pzz -= om * nzz;
     Pxy -= om * nxy; Pxz -= om * nxz; Pyz -= om * nyz;
+    if constexpr (DEV) {
+        const R r3 = dr * third;  // P~_aa = P_aa - 1/3 - dr/3
+        Pxx -= r3; Pyy -= r3; Pzz -= r3;
+    }
     const R tr = Pxx + Pyy + Pzz;
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
@@ -604,7 +664,8 @@ __device__ __forceinline__ void collide_mrt(R (&f)[27], const Moments<R>& M, R u
         if (cx * cz > 0) cPc += R(2) * Pxz; else if (cx * cz < 0) cPc -= R(2) * Pxz;
         if (cy * cz > 0) cPc += R(2) * Pyz; else if (cy * cz < 0) cPc -= R(2) * Pyz;
         const R cu = dot_c(q, ux, uy, uz);
-        f[q] = wgt<R>(q) * (rho * (R(2.5) - R(1.5) * R(c2) + R(3) * cu) + R(4.5) * cPc - R(1.5) * tr);
+        if constexpr (DEV) f[q] = wgt<R>(q) * (dr + R(3) * rho * cu + R(4.5) * cPc - R(1.5) * tr);
+        else f[q] = wgt<R>(q) * (rho * (R(2.5) - R(1.5) * R(c2) + R(3) * cu) + R(4.5) * cPc - R(1.5) * tr);
     }
 }
 
@@ -638,9 +699,14 @@ __device__ __forceinline__ R kbc_shear(int q, const R (&M2)[6], const R (&M3)[7]
 // computed earlier from the same inputs
 __device__ __forceinline__ void opaque(double& v) { asm volatile("" : "+v"(v)); }
 
-template <class R>
-__device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R ux, R uy, R uz, R Fx, R Fy, R Fz, R nu)
+// DEV (FP32 only): f holds f - w (in and out), M the moments of the
+// deviations (rho full, P_aa less 1/3), dr = rho - 1; the equilibrium enters
+// as EqDev's deviations, and in full only where it weighs (1/f_eq, source)
+template <class R, bool DEV = false>
+__device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R dr, R ux, R uy, R uz, R Fx, R Fy, R Fz,
+                                            R nu)
 {
+    static_assert(!DEV || sizeof(R) == 4, "deviation form at FP32 only");
     const R rho = M.rho;
     R px[3], py[3], pz[3];
     phi3(ux, px); phi3(uy, py); phi3(uz, pz);
@@ -660,18 +726,23 @@ __device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R u
     }
     const R third = R(1) / R(3);
     // non-equilibrium parts of the moments: s is linear in them
-    R M2[6] = {M.pxx - rho * (third + ux * ux), M.pyy - rho * (third + uy * uy), M.pzz - rho * (third + uz * uz),
+    const R r3 = DEV ? dr * third : R(0);
+    R M2[6] = {DEV ? M.pxx - r3 - rho * ux * ux : M.pxx - rho * (third + ux * ux),
+               DEV ? M.pyy - r3 - rho * uy * uy : M.pyy - rho * (third + uy * uy),
+               DEV ? M.pzz - r3 - rho * uz * uz : M.pzz - rho * (third + uz * uz),
                M.pxy - rho * ux * uy, M.pxz - rho * ux * uz, M.pyz - rho * uy * uz};
     M3[0] -= rho * ux * (third + uy * uy); M3[1] -= rho * ux * (third + uz * uz);
     M3[2] -= rho * uy * (third + ux * ux); M3[3] -= rho * uy * (third + uz * uz);
     M3[4] -= rho * uz * (third + ux * ux); M3[5] -= rho * uz * (third + uy * uy);
     M3[6] -= rho * ux * uy * uz;
+    const EqDev<R> E(rho, dr, ux, uy, uz);  // used under DEV only
     R num = 0, den = 0;
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
-        const R feq = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
+        const R de = DEV ? E(q) : R(0);
+        const R feq = DEV ? de + wgt<R>(q) : rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
         const R ds = kbc_shear<R>(q, M2, M3);
-        const R dh = f[q] - feq - ds;
+        const R dh = DEV ? f[q] - de - ds : f[q] - feq - ds;
         const R ifeq = Fm<R>::rcp(feq);
         num += ds * dh * ifeq;
         den += dh * dh * ifeq;
@@ -711,9 +782,10 @@ __device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R u
     } else {
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
-            const R feq = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
+            const R de = DEV ? E(q) : R(0);
+            const R feq = DEV ? de + wgt<R>(q) : rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
             const R ds = kbc_shear<R>(q, M2, M3);
-            const R dh = f[q] - feq - ds;
+            const R dh = DEV ? f[q] - de - ds : f[q] - feq - ds;
             f[q] -= beta * (R(2) * ds + gamma * dh) - src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
         }
     }
@@ -914,9 +986,9 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
     const int i = lane & (kBX - 1), j = (lane >> kSX) & (kBY - 1), k = lane >> (kSX + kSY);
     const uint32_t own = uint32_t(t[13]) + uint32_t(lane);
 
-    // the cumulant on two-byte storage runs on the stored deviations
-    // f_q - w_q from load to store (collide_cumulant<R, true>)
-    constexpr bool kDev = kBalanced<S> && MODEL == M_CUM;
+    // two-byte storage runs every collision on the stored deviations f_q - w_q
+    // from load to store (the DEV forms of the collide_* functions)
+    constexpr bool kDev = kBalanced<S>;
     R f[27];
     R rho = R(1), ux = R(0), uy = R(0), uz = R(0);
     if (fluid) {
@@ -960,9 +1032,9 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
         if (C.cy_mode == 1) nu = neq_viscosity<R>(M, ux, uy, uz, C);
         const R om = Fm<R>::rcp(R(3) * nu + R(0.5));
         if constexpr (MODEL == M_CUM) collide_cumulant<R, kDev>(f, M, dr, ux, uy, uz, om);
-        else if constexpr (MODEL == M_BGK) collide_bgk<R>(f, rho, ux, uy, uz, Fx, Fy, Fz, om);
-        else if constexpr (MODEL == M_MRT) collide_mrt<R>(f, M, ux, uy, uz, nu);
-        else collide_kbc<R>(f, M, ux, uy, uz, Fx, Fy, Fz, nu);
+        else if constexpr (MODEL == M_BGK) collide_bgk<R, kDev>(f, rho, dr, ux, uy, uz, Fx, Fy, Fz, om);
+        else if constexpr (MODEL == M_MRT) collide_mrt<R, kDev>(f, M, dr, ux, uy, uz, nu);
+        else collide_kbc<R, kDev>(f, M, dr, ux, uy, uz, Fx, Fy, Fz, nu);
     } else if (bc) {
         // sim_tcpc.cu:242-273: the populations entering the partner cell one
         // plane inwards give rho (inflow) or u (outflow); the partner and any

@@ -34,9 +34,9 @@ STEPS = 2000
 TOL = {_abi.F32: (1e-4, 1e-4, 2e-4), _abi.F16_OFFSET: (1e-3, 1e-3, 2e-3), _abi.BF16_OFFSET: (5e-3, 5e-3, 1e-2)}
 
 
-def _case(storage):
+def _case(storage, collision=_abi.CUM):
     import bench
-    g, kw = bench.make_case(0.25, storage, _abi.CUM, 0.01)
+    g, kw = bench.make_case(0.25, storage, collision, 0.01)
     kw.update(pulse_amp=0.3, pulse_period=400.0)
     return g, kw
 
@@ -87,6 +87,48 @@ def test_storage_against_fp64_oracle(gpu_lib, reference, storage):
     assert err_g < tg, (name, err_g)
     assert err_split < 0.5, (name, err_split)
     # the flow moved: the comparison is not of two rest states
+    assert np.abs(uref).max() > 0.01
+
+
+@pytest.fixture(scope="module")
+def model_reference():
+    """FP64 oracle runs of the same pulsatile junction for the other three
+    collision models, made on first use."""
+    cache = {}
+
+    def get(collision):
+        if collision not in cache:
+            g, kw = _case(_abi.F64, collision)
+            o = OracleCase(g.flat(), g.dims, **kw)
+            o.step(STEPS)
+            cache[collision] = {k: o.field(k) for k in ("rho", "vx", "vy", "vz", "label")}
+        return cache[collision]
+    return get
+
+
+@pytest.mark.parametrize("storage", [_abi.F32, _abi.F16_OFFSET])
+@pytest.mark.parametrize("collision", [_abi.BGK, _abi.MRT, _abi.KBC])
+def test_models_storage_against_fp64_oracle(gpu_lib, model_reference, collision, storage):
+    """BGK, MRT_LES and KBC_N4 on the same junction (CY-A neq, pulsatile, 2,000
+    steps), stored as FP32 and FP16-offset, against the FP64 oracle: relative
+    L2 of velocity and of rho - 1 within 1e-4 (FP32) and 1e-3 (FP16-offset).
+    Measured on the MI355X (velocity / rho - 1): FP32 BGK 1.1e-5 / 8.5e-6, MRT
+    1.6e-5 / 1.7e-5, KBC 1.0e-5 / 8.0e-6.  Before the equilibrium's 1-D
+    factors summed to exactly 1 in FP32, BGK and KBC missed by 2.9e-4 here
+    (and by 5e-3 after 40,000 steps), which the 1e-4 bound rejects."""
+    ref = model_reference(collision)
+    g, kw = _case(storage, collision)
+    c = Case(g.flat(), g.dims, **kw)
+    c.step(STEPS)
+    act = ref["label"] != 1
+    u = np.stack([c.field(k)[act] for k in ("vx", "vy", "vz")])
+    uref = np.stack([ref[k][act] for k in ("vx", "vy", "vz")])
+    err_u = rel_l2(u, uref)
+    err_rho = rel_l2(c.field("rho")[act] - 1.0, ref["rho"][act] - 1.0)
+    name = f"{_abi.STORAGE_NAMES[storage]} collision {collision}"
+    print(f"{name}: u {err_u:.3e} rho-1 {err_rho:.3e}")
+    tol = 1e-4 if storage == _abi.F32 else 1e-3
+    assert err_u < tol and err_rho < tol, (name, err_u, err_rho)
     assert np.abs(uref).max() > 0.01
 
 
