@@ -624,6 +624,20 @@ __device__ __forceinline__ void collide_bgk(R (&f)[27], R rho, R dr, R ux, R uy,
 
 // regularised collision with a Smagorinsky rate, "MRT_LES"
 // (lbm_col_mrt.h:14-77); the force is not applied, as in the reference
+// sum_q w_q (5/2 - 3/2 c_q^2) - 1 (a = 0) and 3 sum_q w_q c_qx^2 - 1 (a = 1)
+// over the lattice weights as rounded to R: the mass and momentum that one
+// MRT reconstruction gains per unit of rho and rho u (2^-27 each in FP32)
+template <class R>
+__host__ __device__ constexpr double mrt_excess(int a)
+{
+    double s = 0.0;
+    for (int q = 0; q < 27; ++q) {
+        const int cx = q / 9 - 1, c2 = cx * cx + ((q / 3) % 3 - 1) * ((q / 3) % 3 - 1) + (q % 3 - 1) * (q % 3 - 1);
+        s += double(R(lattice_weight(q))) * (a == 0 ? 2.5 - 1.5 * c2 : 3.0 * cx * cx);
+    }
+    return s - 1.0;
+}
+
 // DEV: f holds f - w (in and out) and M the moments of the deviations (rho
 // full, P_aa less 1/3), dr = rho - 1.  With P~ = P - rho/3 I the relaxed
 // moments less their isotropic part, f_q - w_q = w_q (dr + 3 rho c.u +
@@ -664,8 +678,22 @@ __device__ __forceinline__ void collide_mrt(R (&f)[27], const Moments<R>& M, R d
         if (cx * cz > 0) cPc += R(2) * Pxz; else if (cx * cz < 0) cPc -= R(2) * Pxz;
         if (cy * cz > 0) cPc += R(2) * Pyz; else if (cy * cz < 0) cPc -= R(2) * Pyz;
         const R cu = dot_c(q, ux, uy, uz);
-        if constexpr (DEV) f[q] = wgt<R>(q) * (dr + R(3) * rho * cu + R(4.5) * cPc - R(1.5) * tr);
-        else f[q] = wgt<R>(q) * (rho * (R(2.5) - R(1.5) * R(c2) + R(3) * cu) + R(4.5) * cPc - R(1.5) * tr);
+        if constexpr (DEV) {
+            f[q] = wgt<R>(q) * (dr + R(3) * rho * cu + R(4.5) * cPc - R(1.5) * tr);
+        } else {
+            // full populations: the rounded weights' excess mass (rest
+            // direction) and momentum (the six faces, half each) is taken out
+            // inside the one rounding of the fused multiply-add; outside it,
+            // 7e-9 rho is below the spacing of FP32 near w_q and the excess
+            // would accumulate, step after step (40,000-step junction:
+            // velocity 2.6e-4 off FP64 before)
+            constexpr R em = R(mrt_excess<R>(0)), ej = R(0.5 * mrt_excess<R>(1));
+            const R br = rho * (R(2.5) - R(1.5) * R(c2) + R(3) * cu) + R(4.5) * cPc - R(1.5) * tr;
+            R corr = R(0);
+            if (c2 == 0) corr = -em * rho;
+            else if (c2 == 1) corr = -ej * rho * cu;
+            f[q] = fma(wgt<R>(q), br, corr);
+        }
     }
 }
 
