@@ -574,6 +574,19 @@ __device__ __forceinline__ void eq_dev3(R u, R (&e)[3])
     e[0] = R(3) * (uu - u);
     e[2] = R(3) * (uu + u);
 }
+// sum_q w_q - 1 (a = 0) and 3 sum_q w_q c_qx^2 - 1 (a = 1) over the lattice
+// weights as rounded to R: 2^-27 each in FP32
+template <class R>
+__host__ __device__ constexpr double weight_excess(int a)
+{
+    double s = 0.0;
+    for (int q = 0; q < 27; ++q) {
+        const int cx = q / 9 - 1;
+        s += double(R(lattice_weight(q))) * (a == 0 ? 1.0 : 3.0 * cx * cx);
+    }
+    return s - 1.0;
+}
+
 // With e_xy = (1 + e_x)(1 + e_y) - 1 per (x, y) pair, rho - 1 + rho E_q =
 // A + B e_z with A = rho - 1 + rho e_xy and B = rho (1 + e_xy): two
 // operations per direction once the nine pairs are formed
@@ -595,21 +608,43 @@ struct EqDev {
     }
     // f_eq,q - w_q
     __device__ R operator()(int q) const { return wgt<R>(q) * (A[q / 3] + B[q / 3] * ez[q % 3]); }
+    // the same over the rounded weights, whose sum and momentum sums exceed 1
+    // by em and 2 ej (weight_excess): the rest direction gives back em times
+    // the mass it rests on (base: rho for populations stored in full over the
+    // rounded weights, rho - 1 for deviations), each face ej rho (c.u).  The
+    // amounts (7e-9 rho at FP32) are far below the spacing of FP32 near w_q,
+    // so this is only possible on the deviation
+    __device__ R fixed(int q, R base, R rho, R cu) const
+    {
+        constexpr R em = R(weight_excess<R>(0)), ej = R(0.5 * weight_excess<R>(1));
+        const int c2 = qx(q) * qx(q) + qy(q) * qy(q) + qz(q) * qz(q);
+        R de = (*this)(q);
+        if (c2 == 0) de -= em * base;
+        else if (c2 == 1) de -= ej * rho * cu;
+        return de;
+    }
 };
 
 // improved BGK with product-form equilibrium and source term
-// (lbm_col_bgk.h:14-140).  DEV: f holds f - w (in and out), dr = rho - 1
+// (lbm_col_bgk.h:14-140).  DEV: f holds f - w (in and out), dr = rho - 1.
+// FP32 populations stored in full take the equilibrium as w + EqDev too,
+// with f - w exact (Sterbenz), so that no value near w_q is rounded before
+// the final update (FP32, 40,000-step junction: velocity 1.4e-4 off FP64 with
+// the product form, whose 2/3 - u^2 alone rounds by up to 3e-8); FP64 keeps
+// the product form of the reference
 template <class R, bool DEV = false>
 __device__ __forceinline__ void collide_bgk(R (&f)[27], R rho, R dr, R ux, R uy, R uz, R Fx, R Fy, R Fz, R om)
 {
     const R src = (R(1) - R(0.5) * om) * R(3) * Fm<R>::rcp(rho);
     const R uF = ux * Fx + uy * Fy + uz * Fz;
-    if constexpr (DEV) {
+    if constexpr (DEV || sizeof(R) == 4) {
         const EqDev<R> E(rho, dr, ux, uy, uz);
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
-            const R de = E(q);
-            f[q] += (de - f[q]) * om + src * (dot_c(q, Fx, Fy, Fz) - uF) * (de + wgt<R>(q));
+            const R w = wgt<R>(q);
+            const R de = E.fixed(q, DEV ? dr : rho, rho, dot_c(q, ux, uy, uz));
+            const R d = DEV ? f[q] : f[q] - w;
+            f[q] += (de - d) * om + src * (dot_c(q, Fx, Fy, Fz) - uF) * (de + w);
         }
     } else {
         R px[3], py[3], pz[3];
@@ -729,7 +764,9 @@ __device__ __forceinline__ void opaque(double& v) { asm volatile("" : "+v"(v)); 
 
 // DEV (FP32 only): f holds f - w (in and out), M the moments of the
 // deviations (rho full, P_aa less 1/3), dr = rho - 1; the equilibrium enters
-// as EqDev's deviations, and in full only where it weighs (1/f_eq, source)
+// as EqDev's deviations, and in full only where it weighs (1/f_eq, source).
+// FP32 populations stored in full take the equilibrium from EqDev as well (as
+// collide_bgk does); FP64 keeps the product form
 template <class R, bool DEV = false>
 __device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R dr, R ux, R uy, R uz, R Fx, R Fy, R Fz,
                                             R nu)
@@ -763,14 +800,15 @@ __device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R d
     M3[2] -= rho * uy * (third + ux * ux); M3[3] -= rho * uy * (third + uz * uz);
     M3[4] -= rho * uz * (third + ux * ux); M3[5] -= rho * uz * (third + uy * uy);
     M3[6] -= rho * ux * uy * uz;
-    const EqDev<R> E(rho, dr, ux, uy, uz);  // used under DEV only
+    constexpr bool kEq = DEV || sizeof(R) == 4;  // equilibrium from EqDev
+    const EqDev<R> E(rho, dr, ux, uy, uz);
     R num = 0, den = 0;
 #pragma unroll
     for (int q = 0; q < 27; ++q) {
-        const R de = DEV ? E(q) : R(0);
-        const R feq = DEV ? de + wgt<R>(q) : rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
+        const R de = kEq ? E.fixed(q, DEV ? dr : rho, rho, dot_c(q, ux, uy, uz)) : R(0);
+        const R feq = kEq ? de + wgt<R>(q) : rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
         const R ds = kbc_shear<R>(q, M2, M3);
-        const R dh = DEV ? f[q] - de - ds : f[q] - feq - ds;
+        const R dh = DEV ? f[q] - de - ds : kEq ? (f[q] - wgt<R>(q)) - de - ds : f[q] - feq - ds;
         const R ifeq = Fm<R>::rcp(feq);
         num += ds * dh * ifeq;
         den += dh * dh * ifeq;
@@ -810,10 +848,10 @@ __device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R d
     } else {
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
-            const R de = DEV ? E(q) : R(0);
-            const R feq = DEV ? de + wgt<R>(q) : rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
+            const R de = E.fixed(q, DEV ? dr : rho, rho, dot_c(q, ux, uy, uz));
+            const R feq = de + wgt<R>(q);
             const R ds = kbc_shear<R>(q, M2, M3);
-            const R dh = DEV ? f[q] - de - ds : f[q] - feq - ds;
+            const R dh = DEV ? f[q] - de - ds : (f[q] - wgt<R>(q)) - de - ds;
             f[q] -= beta * (R(2) * ds + gamma * dh) - src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
         }
     }
