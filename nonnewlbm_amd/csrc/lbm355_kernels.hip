@@ -632,7 +632,9 @@ struct EqDev {
 // the final update (FP32, 40,000-step junction: velocity 1.4e-4 off FP64 with
 // the product form, whose 2/3 - u^2 alone rounds by up to 3e-8); FP64 keeps
 // the product form of the reference
-template <class R, bool DEV = false>
+// FORCE: the case carries a body force (the fd-force Carreau-Yasuda mode);
+// without one the source term is zero and is not formed
+template <class R, bool DEV = false, bool FORCE = true>
 __device__ __forceinline__ void collide_bgk(R (&f)[27], R rho, R dr, R ux, R uy, R uz, R Fx, R Fy, R Fz, R om)
 {
     const R src = (R(1) - R(0.5) * om) * R(3) * Fm<R>::rcp(rho);
@@ -644,7 +646,8 @@ __device__ __forceinline__ void collide_bgk(R (&f)[27], R rho, R dr, R ux, R uy,
             const R w = wgt<R>(q);
             const R de = E.fixed(q, DEV ? dr : rho, rho, dot_c(q, ux, uy, uz));
             const R d = DEV ? f[q] : f[q] - w;
-            f[q] += (de - d) * om + src * (dot_c(q, Fx, Fy, Fz) - uF) * (de + w);
+            f[q] += (de - d) * om;
+            if constexpr (FORCE) f[q] += src * (dot_c(q, Fx, Fy, Fz) - uF) * (de + w);
         }
     } else {
         R px[3], py[3], pz[3];
@@ -652,7 +655,8 @@ __device__ __forceinline__ void collide_bgk(R (&f)[27], R rho, R dr, R ux, R uy,
 #pragma unroll
         for (int q = 0; q < 27; ++q) {
             const R feq = rho * px[q / 9] * py[(q / 3) % 3] * pz[q % 3];
-            f[q] += (feq - f[q]) * om + src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
+            if constexpr (FORCE) f[q] += (feq - f[q]) * om + src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
+            else f[q] += (feq - f[q]) * om;
         }
     }
 }
@@ -767,7 +771,7 @@ __device__ __forceinline__ void opaque(double& v) { asm volatile("" : "+v"(v)); 
 // as EqDev's deviations, and in full only where it weighs (1/f_eq, source).
 // FP32 populations stored in full take the equilibrium from EqDev as well (as
 // collide_bgk does); FP64 keeps the product form
-template <class R, bool DEV = false>
+template <class R, bool DEV = false, bool FORCE = true>
 __device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R dr, R ux, R uy, R uz, R Fx, R Fy, R Fz,
                                             R nu)
 {
@@ -843,7 +847,8 @@ __device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R d
             const R feq = r2 * qx3[q / 9] * qy3[(q / 3) % 3] * qz3[q % 3];
             const R ds = kbc_shear<R>(q, N2, N3);
             const R dh = f[q] - feq - ds;
-            f[q] -= beta * (R(2) * ds + gamma * dh) - src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
+            if constexpr (FORCE) f[q] -= beta * (R(2) * ds + gamma * dh) - src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
+            else f[q] -= beta * (R(2) * ds + gamma * dh);
         }
     } else {
 #pragma unroll
@@ -852,7 +857,8 @@ __device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R d
             const R feq = de + wgt<R>(q);
             const R ds = kbc_shear<R>(q, M2, M3);
             const R dh = DEV ? f[q] - de - ds : (f[q] - wgt<R>(q)) - de - ds;
-            f[q] -= beta * (R(2) * ds + gamma * dh) - src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
+            if constexpr (FORCE) f[q] -= beta * (R(2) * ds + gamma * dh) - src * (dot_c(q, Fx, Fy, Fz) - uF) * feq;
+            else f[q] -= beta * (R(2) * ds + gamma * dh);
         }
     }
 }
@@ -1098,9 +1104,15 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
         if (C.cy_mode == 1) nu = neq_viscosity<R>(M, ux, uy, uz, C);
         const R om = Fm<R>::rcp(R(3) * nu + R(0.5));
         if constexpr (MODEL == M_CUM) collide_cumulant<R, kDev>(f, M, dr, ux, uy, uz, om);
-        else if constexpr (MODEL == M_BGK) collide_bgk<R, kDev>(f, rho, dr, ux, uy, uz, Fx, Fy, Fz, om);
+        else if constexpr (MODEL == M_BGK) {
+            if (C.force) collide_bgk<R, kDev, true>(f, rho, dr, ux, uy, uz, Fx, Fy, Fz, om);
+            else collide_bgk<R, kDev, false>(f, rho, dr, ux, uy, uz, Fx, Fy, Fz, om);
+        }
         else if constexpr (MODEL == M_MRT) collide_mrt<R, kDev>(f, M, dr, ux, uy, uz, nu);
-        else collide_kbc<R, kDev>(f, M, dr, ux, uy, uz, Fx, Fy, Fz, nu);
+        else {
+            if (C.force) collide_kbc<R, kDev, true>(f, M, dr, ux, uy, uz, Fx, Fy, Fz, nu);
+            else collide_kbc<R, kDev, false>(f, M, dr, ux, uy, uz, Fx, Fy, Fz, nu);
+        }
     } else if (bc) {
         // sim_tcpc.cu:242-273: the populations entering the partner cell one
         // plane inwards give rho (inflow) or u (outflow); the partner and any
