@@ -1104,14 +1104,17 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
         if (C.cy_mode == 1) nu = neq_viscosity<R>(M, ux, uy, uz, C);
         const R om = Fm<R>::rcp(R(3) * nu + R(0.5));
         if constexpr (MODEL == M_CUM) collide_cumulant<R, kDev>(f, M, dr, ux, uy, uz, om);
+        // BGK and KBC without a body force drop the source term at compile
+        // time (FP32 arithmetic only: a second copy of the FP64 KBC, at 256
+        // VGPRs with scratch, halves its speed)
         else if constexpr (MODEL == M_BGK) {
-            if (C.force) collide_bgk<R, kDev, true>(f, rho, dr, ux, uy, uz, Fx, Fy, Fz, om);
-            else collide_bgk<R, kDev, false>(f, rho, dr, ux, uy, uz, Fx, Fy, Fz, om);
+            if (sizeof(R) == 8 || C.force) collide_bgk<R, kDev, true>(f, rho, dr, ux, uy, uz, Fx, Fy, Fz, om);
+            else if constexpr (sizeof(R) == 4) collide_bgk<R, kDev, false>(f, rho, dr, ux, uy, uz, Fx, Fy, Fz, om);
         }
         else if constexpr (MODEL == M_MRT) collide_mrt<R, kDev>(f, M, dr, ux, uy, uz, nu);
         else {
-            if (C.force) collide_kbc<R, kDev, true>(f, M, dr, ux, uy, uz, Fx, Fy, Fz, nu);
-            else collide_kbc<R, kDev, false>(f, M, dr, ux, uy, uz, Fx, Fy, Fz, nu);
+            if (sizeof(R) == 8 || C.force) collide_kbc<R, kDev, true>(f, M, dr, ux, uy, uz, Fx, Fy, Fz, nu);
+            else if constexpr (sizeof(R) == 4) collide_kbc<R, kDev, false>(f, M, dr, ux, uy, uz, Fx, Fy, Fz, nu);
         }
     } else if (bc) {
         // sim_tcpc.cu:242-273: the populations entering the partner cell one
