@@ -54,7 +54,9 @@ enum {
 /* population storage; arithmetic is FP64 for F64 and FP32 for the others */
 enum {
     LBM355_F64 = 0,         /* TraitsDP (solver/lbm3d/defs.h:141)                */
-    LBM355_F32 = 1,         /* TraitsSP, the sim_tcpc default (defs.h:140)       */
+    LBM355_F32 = 1,         /* TraitsSP, the sim_tcpc default (defs.h:140);
+                               stored as f_q - w_q, so that the small
+                               non-equilibrium part keeps its 24 bits           */
     LBM355_F16_OFFSET = 2,  /* (f_q - w_q) 2^e_q stored as IEEE half (w_q 2^e_q =
                                16/27), stochastically rounded, FP32 arithmetic  */
     LBM355_BF16_OFFSET = 3  /* f_q - w_q stored as bfloat16, stochastically

@@ -813,6 +813,30 @@ void put_float(std::vector<char>& buf, float f)
     put_be32(buf, u);
 }
 
+
+// fn(offset, q, dense index q * N + cell) for every population the AA buffer
+// holds for the case's current step parity: the post-collision f_q of a cell
+// is pair (x + c_q, q) after an odd step and pair (x, opp q) after an even
+// one; wall cells next to the flow hold the populations in flight through
+// full-way bounce-back, the other wall cells hold nothing
+template <class Fn>
+void for_each_held(const lbm355_case& c, Fn&& fn)
+{
+    const int X = c.d.nx, Y = c.d.ny, Z = c.d.nz;
+    const int64_t N = int64_t(X) * Y * Z;
+    const bool even_next = (c.steps & 1) == 0;  // the next step is even: the last one was odd
+    for (int z = 0; z < Z; ++z)
+        for (int y = 0; y < Y; ++y)
+            for (int x = 0; x < X; ++x) {
+                const int64_t i = c.cell_index(x, y, z);
+                if (c.lab.map[i] == L_WALL && c.stored_brick(x, y, z) < 0) continue;
+                for (int q = 0; q < kQ; ++q) {
+                    const int32_t off = even_next ? c.pair_offset(x + qx(q), y + qy(q), z + qz(q), q)
+                                                  : c.pair_offset(x, y, z, 26 - q);
+                    if (off >= 0) fn(off, q, int64_t(q) * N + i);
+                }
+            }
+}
 }  // namespace
 
 // ===========================================================================
@@ -1146,8 +1170,7 @@ int lbm355_case_get_field(lbm355_case* c, int field, double* host, size_t bytes)
 int lbm355_case_get_populations(lbm355_case* c, double* host, size_t bytes)
 {
     if (!c || !host) return fail(LBM355_EINVAL, "null argument");
-    const int X = c->d.nx, Y = c->d.ny, Z = c->d.nz;
-    const int64_t N = int64_t(X) * Y * Z;
+    const int64_t N = int64_t(c->d.nx) * c->d.ny * c->d.nz;
     if (bytes < size_t(N) * kQ * 8) return fail(LBM355_EINVAL, "buffer too small");
     const size_t sb = storage_bytes(c->storage);
     const size_t ne = size_t(c->n_stored + 1) * kQ * kLanes;
@@ -1157,7 +1180,7 @@ int lbm355_case_get_populations(lbm355_case* c, double* host, size_t bytes)
     auto value = [&](int32_t off, int q) -> double {
         switch (c->storage) {
         case LBM355_F64: { double v; std::memcpy(&v, &raw[size_t(off) * 8], 8); return v; }
-        case LBM355_F32: { float v; std::memcpy(&v, &raw[size_t(off) * 4], 4); return v; }
+        case LBM355_F32: { float v; std::memcpy(&v, &raw[size_t(off) * 4], 4); return double(v) + lattice_weight(q); }
         case LBM355_F16_OFFSET: {
             uint16_t h; std::memcpy(&h, &raw[size_t(off) * 2], 2);
             const uint32_t sgn = uint32_t(h & 0x8000) << 16;
@@ -1180,23 +1203,7 @@ int lbm355_case_get_populations(lbm355_case* c, double* host, size_t bytes)
         }
     };
     std::fill(host, host + N * kQ, 0.0);
-    const bool even_next = (c->steps & 1) == 0;  // the next step is even: the last one was odd
-    for (int z = 0; z < Z; ++z)
-        for (int y = 0; y < Y; ++y)
-            for (int x = 0; x < X; ++x) {
-                const int64_t i = c->cell_index(x, y, z);
-                // wall cells next to the flow hold the populations in flight
-                // through full-way bounce-back; the others hold nothing
-                if (c->lab.map[i] == L_WALL && c->stored_brick(x, y, z) < 0) continue;
-                for (int q = 0; q < kQ; ++q) {
-                    // post-collision f_q of the cell: pair (x + c_q, q) after an
-                    // odd step, pair (x, opp q) after an even one
-                    const int32_t off = even_next ? c->pair_offset(x + qx(q), y + qy(q), z + qz(q), q)
-                                                  : c->pair_offset(x, y, z, 26 - q);
-                    if (off < 0) continue;
-                    host[int64_t(q) * N + i] = value(off, q);
-                }
-            }
+    for_each_held(*c, [&](int32_t off, int q, int64_t k) { host[k] = value(off, q); });
     return LBM355_OK;
 }
 
@@ -1358,7 +1365,7 @@ const int kMacroMap[kMacroTcpc] = {
     LBM355_FIELD_S11, LBM355_FIELD_S12, LBM355_FIELD_S13, LBM355_FIELD_S22, LBM355_FIELD_S32, LBM355_FIELD_S33};
 
 template <class T>
-int save_arrays(lbm355_case* c, const char* dir, const std::vector<double>& pops)
+int save_arrays(lbm355_case* c, const char* dir, const std::vector<double>& pops, const std::vector<float>& dev)
 {
     const int64_t X = c->d.nx, Y = c->d.ny, Z = c->d.nz, N = X * Y * Z;
     // df: the reference's lat layout x, q, z, y (y fastest, defs.h:57-58)
@@ -1373,6 +1380,18 @@ int save_arrays(lbm355_case* c, const char* dir, const std::vector<double>& pops
     // reads is df_(iterations % 2)
     if ((rc = write_raw(state_file(dir, "df_0"), df))) return rc;
     if ((rc = write_raw(state_file(dir, "df_1"), df))) return rc;
+    if (!dev.empty()) {
+        // an FP32 case's stored words f - w (f32_words), in the same layout,
+        // for a bit-exact restart; a reader of the reference layout ignores
+        // the extra file
+        std::vector<float> out(size_t(N) * kQ);
+        k = 0;
+        for (int64_t x = 0; x < X; ++x)
+            for (int q = 0; q < kQ; ++q)
+                for (int64_t z = 0; z < Z; ++z)
+                    for (int64_t y = 0; y < Y; ++y) out[k++] = dev[size_t(q * N + x + X * (y + Y * z))];
+        if ((rc = write_raw(state_file(dir, "df_dev"), out))) return rc;
+    }
     std::vector<int16_t> map(static_cast<size_t>(N));
     k = 0;
     for (int64_t x = 0; x < X; ++x)
@@ -1394,7 +1413,7 @@ int save_arrays(lbm355_case* c, const char* dir, const std::vector<double>& pops
 
 template <class T>
 int load_arrays(lbm355_case* c, const char* dir, int64_t iterations, std::vector<double>& pops,
-                std::vector<std::vector<double>>& fields)
+                std::vector<std::vector<double>>& fields, std::vector<float>& dev)
 {
     const int64_t X = c->d.nx, Y = c->d.ny, Z = c->d.nz, N = X * Y * Z;
     std::vector<T> df(size_t(N) * kQ);
@@ -1406,6 +1425,22 @@ int load_arrays(lbm355_case* c, const char* dir, int64_t iterations, std::vector
         for (int q = 0; q < kQ; ++q)
             for (int64_t z = 0; z < Z; ++z)
                 for (int64_t y = 0; y < Y; ++y) pops[size_t(q * N + x + X * (y + Y * z))] = double(df[k++]);
+    // an FP32 case's own checkpoint also holds its stored words f - w
+    // (save_arrays); a reference checkpoint has only f, from which the
+    // deviations follow to FP32 rounding of f
+    dev.clear();
+    const std::string devf = state_file(dir, "df_dev");
+    struct stat dst;
+    if (c->storage == LBM355_F32 && ::stat(devf.c_str(), &dst) == 0) {
+        std::vector<float> in(size_t(N) * kQ);
+        if ((rc = read_raw(devf, in))) return rc;
+        dev.assign(size_t(N) * kQ, 0.0f);
+        k = 0;
+        for (int64_t x = 0; x < X; ++x)
+            for (int q = 0; q < kQ; ++q)
+                for (int64_t z = 0; z < Z; ++z)
+                    for (int64_t y = 0; y < Y; ++y) dev[size_t(q * N + x + X * (y + Y * z))] = in[k++];
+    }
     std::vector<T> macro(size_t(N) * kMacroTcpc);
     if ((rc = read_raw(state_file(dir, "macro"), macro))) return rc;
     fields.assign(kMacroTcpc, std::vector<double>());
@@ -1442,8 +1477,6 @@ uint16_t host_bf16(float v)
 // them) into the AA buffer for the case's current step parity
 int set_populations(lbm355_case* c, const std::vector<double>& pops)
 {
-    const int X = c->d.nx, Y = c->d.ny, Z = c->d.nz;
-    const int64_t N = int64_t(X) * Y * Z;
     const size_t sb = storage_bytes(c->storage);
     const size_t ne = size_t(c->n_stored + 1) * kQ * kLanes;
     std::vector<unsigned char> raw(ne * sb);
@@ -1452,7 +1485,7 @@ int set_populations(lbm355_case* c, const std::vector<double>& pops)
     auto put = [&](int32_t off, int q, double v) {
         switch (c->storage) {
         case LBM355_F64: std::memcpy(&raw[size_t(off) * 8], &v, 8); break;
-        case LBM355_F32: { const float f = float(v); std::memcpy(&raw[size_t(off) * 4], &f, 4); break; }
+        case LBM355_F32: { const float f = float(v - lattice_weight(q)); std::memcpy(&raw[size_t(off) * 4], &f, 4); break; }
         case LBM355_F16_OFFSET: {
             const uint16_t h = host_f16((float(v) - float(lattice_weight(q))) * float(1 << half_scale_exp(q)));
             std::memcpy(&raw[size_t(off) * 2], &h, 2);
@@ -1464,20 +1497,29 @@ int set_populations(lbm355_case* c, const std::vector<double>& pops)
         }
         }
     };
-    const bool even_next = (c->steps & 1) == 0;
-    for (int z = 0; z < Z; ++z)
-        for (int y = 0; y < Y; ++y)
-            for (int x = 0; x < X; ++x) {
-                const int64_t i = c->cell_index(x, y, z);
-                if (c->lab.map[i] == L_WALL && c->stored_brick(x, y, z) < 0) continue;
-                for (int q = 0; q < kQ; ++q) {
-                    const int32_t off = even_next ? c->pair_offset(x + qx(q), y + qy(q), z + qz(q), q)
-                                                  : c->pair_offset(x, y, z, 26 - q);
-                    if (off < 0) continue;
-                    put(off, q, pops[size_t(int64_t(q) * N + i)]);
-                }
-            }
+    for_each_held(*c, [&](int32_t off, int q, int64_t k) { put(off, q, pops[size_t(k)]); });
     HIP_TRY(hipMemcpy(c->f.p, raw.data(), raw.size(), hipMemcpyHostToDevice));
+    return LBM355_OK;
+}
+
+// the stored FP32 words (f - w) of the held slots, dense [q][cell], read
+// (Write = false) or written back.  An FP32 checkpoint keeps them beside the
+// reference layout's float f: w + d is not exact in a double once d is small
+// (d = 1e-11 has bits below the double spacing near w), so the dense doubles
+// of lbm355_case_get_populations cannot carry them either
+template <bool Write>
+int f32_words(lbm355_case* c, std::vector<float>& dev)
+{
+    const int64_t N = int64_t(c->d.nx) * c->d.ny * c->d.nz;
+    std::vector<float> raw(size_t(c->n_stored + 1) * kQ * kLanes);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(raw.data(), c->f.p, raw.size() * 4, hipMemcpyDeviceToHost));
+    if (!Write) dev.assign(size_t(N) * kQ, 0.0f);
+    for_each_held(*c, [&](int32_t off, int, int64_t k) {
+        if (Write) raw[size_t(off)] = dev[size_t(k)];
+        else dev[size_t(k)] = raw[size_t(off)];
+    });
+    if (Write) HIP_TRY(hipMemcpy(c->f.p, raw.data(), raw.size() * 4, hipMemcpyHostToDevice));
     return LBM355_OK;
 }
 
@@ -1511,7 +1553,9 @@ int lbm355_case_save_state(lbm355_case* c, const char* dir, const lbm355_state_d
     if (!fp) return fail(LBM355_EINVAL, "cannot create lbm355 state");
     std::fprintf(fp, "%d\n%d\n%ld\n%ld\n", c->storage, c->model, long(s->stat_prev_iter), long(c->stats_steps));
     std::fclose(fp);
-    return c->storage == LBM355_F64 ? save_arrays<double>(c, dir, pops) : save_arrays<float>(c, dir, pops);
+    std::vector<float> dev;
+    if (c->storage == LBM355_F32 && (rc = f32_words<false>(c, dev))) return rc;
+    return c->storage == LBM355_F64 ? save_arrays<double>(c, dir, pops, dev) : save_arrays<float>(c, dir, pops, dev);
 }
 
 int lbm355_case_load_state(lbm355_case* c, const char* dir, lbm355_state_desc* s)
@@ -1552,11 +1596,13 @@ int lbm355_case_load_state(lbm355_case* c, const char* dir, lbm355_state_desc* s
     int rc;
     // a reference state is dreal = float unless this case stores FP64
     const bool f64 = c->storage == LBM355_F64;
-    rc = f64 ? load_arrays<double>(c, dir, it, pops, fields) : load_arrays<float>(c, dir, it, pops, fields);
+    std::vector<float> dev;
+    rc = f64 ? load_arrays<double>(c, dir, it, pops, fields, dev) : load_arrays<float>(c, dir, it, pops, fields, dev);
     if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->steps = it;
     if ((rc = set_populations(c, pops))) return rc;
+    if (!dev.empty() && (rc = f32_words<true>(c, dev))) return rc;
     // macro rho,u of the saved step, statistics sums, body force, strain
     for (int id = 0; id < kMacroTcpc; ++id) {
         const int f = kMacroMap[id];

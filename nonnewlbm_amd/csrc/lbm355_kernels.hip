@@ -124,28 +124,39 @@ struct Pop<S_F64> {
     __device__ static void put(T* p, int64_t i, int, R v) { p[i] = v; }
 };
 
+// FP32 holds f_q - w_q as well.  At lattice viscosity 1e-4 the
+// non-equilibrium stress that carries the viscosity is about 1e-7 of a
+// population, which FP32 rounds to 3e-8 near w_q: stored in full, the
+// cumulant's velocity on TCPC25_03 (material sx, regime v1) drifted 3.2e-3
+// from FP64 in 20,000 steps.  The deviation keeps 24 bits of its own
 template <>
 struct Pop<S_F32> {
     using T = float;
     using R = float;
     template <int A = 0>
-    __device__ static R ld(rsrc_t r, uint32_t i, int) { return ld_f32<A>(r, i * 4u); }
+    __device__ static R ld(rsrc_t r, uint32_t i, int q) { return ld_f32<A>(r, i * 4u) + wgt<float>(q); }
     template <int A = 0>
-    __device__ static void st(rsrc_t r, uint32_t i, int, R v, uint32_t) { st_f32<A>(r, i * 4u, v); }
-    __device__ static void put(T* p, int64_t i, int, R v) { p[i] = v; }
+    __device__ static R ld_dev(rsrc_t r, uint32_t i, int) { return ld_f32<A>(r, i * 4u); }
+    template <int A = 0>
+    __device__ static void st(rsrc_t r, uint32_t i, int q, R v, uint32_t) { st_f32<A>(r, i * 4u, v - wgt<float>(q)); }
+    template <int A = 0>
+    __device__ static void st_dev(rsrc_t r, uint32_t i, int, R d, uint32_t) { st_f32<A>(r, i * 4u, d); }
+    __device__ static void put(T* p, int64_t i, int q, R v) { p[i] = v - wgt<float>(q); }
 };
 
 // Two-byte storage rounds stochastically: a value is rounded up with the
-// probability of its distance from the lower neighbour, by adding uniform
-// random bits below the kept mantissa and truncating.  The rounding error then
-// has zero mean.  With round-to-nearest the same populations of a steady or
-// periodic flow are rounded the same way step after step, and that bias piles
-// up in the slowly damped modes: on the 512x384x256 junction with pulsatile
-// inflow (cumulant, CY-A, 40,000 steps) velocity missed FP64 by 5.6e-3 (IEEE
-// half) and 0.21 (bfloat16) with round-to-nearest, against 9.2e-4 and 4.3e-3
-// with stochastic rounding; three seeds of the random stream give 9.16e-4 to
-// 9.24e-4 (Croci et al. 2022, "Stochastic rounding: implementation, error
-// analysis and applications", for the method).
+// probability of its distance from the lower neighbour (IEEE half: gfx950's
+// v_cvt_sr_f16_f32 with a random word; bfloat16: uniform random bits added
+// below the kept mantissa, then truncation).  The rounding error then has zero
+// mean.  With round-to-nearest the same populations of a steady or periodic
+// flow are rounded the same way step after step, and that bias piles up in
+// the slowly damped modes: on the 512x384x256 junction with pulsatile inflow
+// (cumulant, CY-A, 40,000 steps) velocity missed FP64 by 5.6e-3 (IEEE half)
+// and 0.21 (bfloat16) with round-to-nearest, against 9.2e-4 and 4.3e-3 with
+// stochastic rounding of full-population arithmetic, and 3.1e-4 (half) once
+// the collisions work on the deviations f - w from load to store (Croci et
+// al. 2022, "Stochastic rounding: implementation, error analysis and
+// applications", for the method).
 //
 // The random bits are a function of (step, brick, lane, q) only, so a case
 // gives the same bits alone, in a batch and across a restart.
@@ -718,7 +729,11 @@ __device__ __forceinline__ void collide_mrt(R (&f)[27], const Moments<R>& M, R d
         if (cy * cz > 0) cPc += R(2) * Pyz; else if (cy * cz < 0) cPc -= R(2) * Pyz;
         const R cu = dot_c(q, ux, uy, uz);
         if constexpr (DEV) {
+            // the rounded weights' momentum excess (2^-27 in FP32) off the
+            // faces; their mass excess here is 2^-27 (rho - 1), negligible
+            constexpr R ej = R(0.5 * mrt_excess<R>(1));
             f[q] = wgt<R>(q) * (dr + R(3) * rho * cu + R(4.5) * cPc - R(1.5) * tr);
+            if (c2 == 1) f[q] -= ej * rho * cu;
         } else {
             // full populations: the rounded weights' excess mass (rest
             // direction) and momentum (the six faces, half each) is taken out
@@ -1058,9 +1073,10 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
     const int i = lane & (kBX - 1), j = (lane >> kSX) & (kBY - 1), k = lane >> (kSX + kSY);
     const uint32_t own = uint32_t(t[13]) + uint32_t(lane);
 
-    // two-byte storage runs every collision on the stored deviations f_q - w_q
-    // from load to store (the DEV forms of the collide_* functions)
-    constexpr bool kDev = kBalanced<S>;
+    // FP32 and two-byte storage run every collision on the stored deviations
+    // f_q - w_q from load to store (the DEV forms of the collide_* functions);
+    // FP64, the parity path, keeps full populations
+    constexpr bool kDev = kBalanced<S> || S == S_F32;
     R f[27];
     R rho = R(1), ux = R(0), uy = R(0), uz = R(0);
     if (fluid) {
@@ -1068,7 +1084,10 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
             balanced_loads<S, kDev>(F, t, own, i, j, k, lane, parity, f);
         } else if (parity == 0) {
 #pragma unroll
-            for (int q = 0; q < 27; ++q) f[q] = P::template ld<kEvenAux>(F, own + uint32_t(q * kLanes), q);
+            for (int q = 0; q < 27; ++q) {
+                if constexpr (kDev) f[q] = P::template ld_dev<kEvenAux>(F, own + uint32_t(q * kLanes), q);
+                else f[q] = P::template ld<kEvenAux>(F, own + uint32_t(q * kLanes), q);
+            }
         } else {
 #pragma unroll
             for (int q = 0; q < 27; ++q) {
@@ -1077,7 +1096,8 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
                 const int nb = pick_brick(t, -cx, -cy, -cz, (si & ~(kBX - 1)) != 0, (sj & ~(kBY - 1)) != 0,
                                           (sk & ~(kBZ - 1)) != 0);
                 const int l2 = brick_lane(si, sj, sk);
-                f[q] = P::ld(F, uint32_t(nb + (26 - q) * kLanes + l2), q);
+                if constexpr (kDev) f[q] = P::ld_dev(F, uint32_t(nb + (26 - q) * kLanes + l2), q);
+                else f[q] = P::ld(F, uint32_t(nb + (26 - q) * kLanes + l2), q);
             }
         }
         R Fx = R(0), Fy = R(0), Fz = R(0);
@@ -1169,7 +1189,8 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
         } else if (parity == 0) {
 #pragma unroll
             for (int q = 0; q < 27; ++q)
-                P::template st<kEvenAux>(F, own + uint32_t((26 - q) * kLanes), q, f[q], rnd_q(H, q));
+                if constexpr (kDev) P::template st_dev<kEvenAux>(F, own + uint32_t((26 - q) * kLanes), q, f[q], rnd_q(H, q));
+                else P::template st<kEvenAux>(F, own + uint32_t((26 - q) * kLanes), q, f[q], rnd_q(H, q));
         } else {
 #pragma unroll
             for (int q = 0; q < 27; ++q) {
@@ -1181,7 +1202,8 @@ __device__ __forceinline__ void step_brick(const CT& C, int64_t launch, int b, i
                 // unconditional: a BC-plane cell's writes that leave the grid
                 // land in an outside lane of its brick or in the sink brick,
                 // neither of which is ever read
-                P::st(F, uint32_t(nb + q * kLanes + l2), q, f[q], rnd_q(H, q));
+                if constexpr (kDev) P::st_dev(F, uint32_t(nb + q * kLanes + l2), q, f[q], rnd_q(H, q));
+                else P::st(F, uint32_t(nb + q * kLanes + l2), q, f[q], rnd_q(H, q));
             }
         }
         // macroscopic output (MacroTcpc::outputMacro, sim_tcpc.cu:337-354)
