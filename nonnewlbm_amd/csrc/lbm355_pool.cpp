@@ -80,6 +80,7 @@ double now_s(std::chrono::steady_clock::time_point t0)
 // 30 G cell updates/s at FP32 / FP16, 64-step ticks) this order gives p50
 // 1.3 s and p99 4.1 s against 1.7 / 4.5 s for FIFO and 1.4 / 7.2 s for
 // processor sharing.
+constexpr double kMinNuHalf = 1e-2;          // smallest lattice viscosity served with FP16-offset storage
 constexpr double kAgeWeight = 1.0;            // seconds of remaining service per second of age
 constexpr int64_t kFillCells = 400000;        // active cells of a tick's set that fill the GPU
 constexpr double kRateF32 = 2.0e10, kRateHalf = 3.0e10;  // nominal cell updates per second
@@ -455,6 +456,12 @@ int lbm355_pool_submit(lbm355_pool* p, const lbm355_case_desc* desc, int64_t ste
     if (desc->storage == LBM355_BF16_OFFSET)
         return pool_fail(LBM355_EINVAL, "bfloat16-offset storage is not served: its velocity error against FP64 "
                                         "(4e-3 after 40,000 steps of the 512x384x256 junction) exceeds the 1e-3 bar; use FP16-offset");
+    // IEEE half resolves a deviation to about 2^-11 of itself; at small
+    // lattice viscosities the non-equilibrium stress falls below that
+    if (desc->storage == LBM355_F16_OFFSET && desc->nu_ref < kMinNuHalf)
+        return pool_fail(LBM355_EINVAL, "FP16-offset storage is served at lattice viscosity 1e-2 and above: below it "
+                                        "the velocity error against FP64 exceeds the 1e-3 bar (TCPC25_03 at 1e-3: 8e-3 "
+                                        "after 20,000 steps; at 1e-4: 1.6e-3 after 1,000); use FP32");
     std::unique_ptr<Request> r(new (std::nothrow) Request());
     if (!r) return pool_fail(LBM355_ENOMEM, "host allocation");
     int64_t fp = 0;

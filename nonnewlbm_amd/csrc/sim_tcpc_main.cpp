@@ -94,7 +94,8 @@ void usage()
               << "  --material cy-a|cy-b|blood-newtonian   benchmark materials\n"
               << "  --collision cumulant|bgk|mrt|kbc       default cumulant\n"
               << "  --precision fp32|fp64|fp16             population storage, default fp32 (fp16: f - w\n"
-              << "                                         in IEEE half, stochastically rounded)\n"
+              << "                                         in IEEE half, stochastically rounded; admitted\n"
+              << "                                         at --lbm-viscosity 1e-2 and above)\n"
               << "  --cy-model neq|off|fd-force            Carreau-Yasuda coupling, default neq; off = what a\n"
               << "                                         single-rank reference run computes, fd-force = the\n"
               << "                                         reference's multi-rank algorithm\n"
@@ -412,6 +413,10 @@ int run(const Options& opt)
     if (opt.precision == "bf16")
         throw std::runtime_error("--precision bf16 is not admitted: bfloat16 storage misses FP64 velocity by "
                                  "4e-3 after 40,000 steps of the 512x384x256 junction, above the 1e-3 bar; use fp16");
+    if (opt.precision == "fp16" && opt.lbm_viscosity < 1e-2)
+        throw std::runtime_error("--precision fp16 is not admitted below --lbm-viscosity 1e-2: half precision does "
+                                 "not resolve the non-equilibrium stress there, and the velocity error against FP64 "
+                                 "exceeds the 1e-3 bar (TCPC25_03 at 1e-4: 1.6e-3 after 1,000 steps); use fp32");
     d.storage = pick(opt.precision, {{"fp32", LBM355_F32}, {"fp64", LBM355_F64}, {"fp16", LBM355_F16_OFFSET}},
                      "precision");
     d.nu_ref = opt.lbm_viscosity;

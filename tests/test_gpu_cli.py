@@ -65,6 +65,21 @@ def test_cli_matches_library_and_writes_outputs(gpu_lib, tmp_path):
     assert row["IVC_cells"] == c.info()["area_ivc"]
 
 
+def test_cli_refuses_half_precision_it_cannot_serve(gpu_lib, tmp_path):
+    """--precision bf16 always, and --precision fp16 below --lbm-viscosity
+    1e-2 (the sim_tcpc studies run 1e-4), exit 1 with the reason; fp16 at
+    1e-2 runs."""
+    g = tcpc_mask(0.25)
+    write_wall_vtk(tmp_path / "geometry.vtk", g)
+    common = ["--material", "newtonian", "--ivc-mls", "0.4", "--svc-mls", "0.2", "--steps", "4", "--print-period", "-1"]
+    r = _run(common + ["--case-id", "b", "--lbm-viscosity", "1e-2", "--precision", "bf16"], tmp_path)
+    assert r.returncode == 1 and "bf16 is not admitted" in r.stderr + r.stdout
+    r = _run(common + ["--case-id", "h", "--lbm-viscosity", "1e-4", "--precision", "fp16"], tmp_path)
+    assert r.returncode == 1 and "fp16 is not admitted below --lbm-viscosity 1e-2" in r.stderr + r.stdout
+    r = _run(common + ["--case-id", "h2", "--lbm-viscosity", "1e-2", "--precision", "fp16"], tmp_path)
+    assert r.returncode == 0, r.stderr + r.stdout
+
+
 def test_cli_zero_area_terminates(gpu_lib, tmp_path):
     g = tcpc_mask(0.25)
     g.labels[:] = 1

@@ -125,6 +125,20 @@ def test_pool_refuses_bfloat16(gpu_lib):
     pool.close()
 
 
+def test_pool_refuses_fp16_at_small_lattice_viscosity(gpu_lib):
+    """FP16-offset storage is served from lattice viscosity 1e-2 up: below it
+    half precision does not resolve the non-equilibrium stress (TCPC25_03 at
+    1e-4: velocity 1.6e-3 off FP64 after 1,000 steps), a documented EINVAL
+    at submit; at 1e-2 the same request is accepted."""
+    g, kw = tcpc_case(0.25, storage=_abi.F16_OFFSET, nu_ref=1e-3)
+    pool = Pool(steps_per_tick=8)
+    with pytest.raises(RuntimeError, match="FP16-offset"):
+        pool.submit(g.flat(), g.dims, 10, **kw)
+    rid = pool.submit(g.flat(), g.dims, 10, **dict(kw, nu_ref=1e-2))
+    assert rid >= 0
+    pool.close()
+
+
 def test_pool_steps_shortest_remaining_work_first(gpu_lib):
     """Three requests of the same junction (about 620k active cells, more than
     one tick's set needs to fill the GPU) submitted together with 30,000,
