@@ -20,7 +20,11 @@ Also reported: each collision model on the same case, a 15-junction batch
 against the same cases one after another, the first --sweep-requests
 requests of the 200-request serving stream (Poisson arrivals at 2/s, a
 240 GB budget) against plain FP32 service of the same requests, and the
-64x32x32 FP64 channel for its full 1,000 steps on the GPU and the CPU.  The
+64x32x32 FP64 channel for its full 1,000 steps on the GPU and the CPU, and
+the storage precision study: the junction scaled to 512x384x256 with
+pulsatile inflow (A = 0.3, T = 4,000 steps), cumulant + CY-A, 40,000 steps
+stored as FP64, FP32, FP16- and BF16-offset, each field's error against the
+FP64 run (about 30 s of GPU time).  The
 CPU baseline is the oracle (a dense A/B OpenMP restatement of the
 reference's Host build) on --cpu-steps steps of the benchmark case, with the
 thread count used and lscpu's physical core count.
@@ -364,7 +368,8 @@ def precision_study(scale: float, steps: int, collision: int | None = None, stor
     model = _abi.CUM if collision is None else collision
     name = {v: k for k, v in _abi.COLLISIONS.items() if k in ("cumulant", "bgk", "mrt_les", "kbc_n4")}[model]
     res: dict = {"workload": f"TCPC junction x{scale} pulsatile (A=0.3, T=4000 steps), {name} + CY-A neq, "
-                             f"{steps} steps", "storages": {}}
+                             f"{steps} steps", "bar": "u rel L2 <= 1e-3 and LPA split within 0.5 % of FP64",
+                 "storages": {}}
     ref = None
     for st in storages or (_abi.F64, _abi.F32, _abi.F16_OFFSET, _abi.BF16_OFFSET):
         g, kw = make_case(scale, st, model, 0.01)
@@ -403,6 +408,9 @@ def precision_study(scale: float, steps: int, collision: int | None = None, stor
                 return float(np.sqrt(np.sum((a - b) ** 2)) / np.sqrt(np.sum(b ** 2)))
             row.update(u_rel_l2=rl2(u, ref[0]), rho_minus_1_rel_l2=rl2(rho1, ref[4]), gamma_dot_rel_l2=rl2(gdot, ref[1]),
                        wss_rel_l2=rl2(wss, ref[2]), split_abs_diff_percent=round(100 * abs(split - ref[3]), 5))
+            # the accuracy bar of reduced storage: velocity within 1e-3
+            # relative L2 and the flow split within 0.5 % of the FP64 run
+            row["meets_bar"] = bool(row["u_rel_l2"] <= 1e-3 and row["split_abs_diff_percent"] <= 0.5)
         res["storages"][_abi.STORAGE_NAMES[st]] = row
         print(f"# precision {name} {_abi.STORAGE_NAMES[st]}: {row}", file=sys.stderr, flush=True)
     return res
@@ -548,8 +556,9 @@ def main() -> int:
     ap.add_argument("--cpu-models", action="store_true", help="CPU baseline for all four collision models")
     ap.add_argument("--no-channel", action="store_true", help="skip the 64x32x32 channel GPU/CPU line")
     ap.add_argument("--no-sweep", action="store_true", help="skip the serving stream")
-    ap.add_argument("--precision-study", type=int, default=0, metavar="STEPS",
-                    help="also run the FP64/FP32/FP16/BF16 storage study for STEPS steps")
+    ap.add_argument("--precision-study", type=int, default=40000, metavar="STEPS",
+                    help="steps of the FP64/FP32/FP16/BF16 storage study on the 512x384x256 pulsatile junction "
+                         "(0: skip)")
     ap.add_argument("--precision-scale", type=float, default=2.0)
     ap.add_argument("--precision-models", type=int, default=0, metavar="STEPS",
                     help="the same study for BGK, MRT_LES and KBC_N4 (FP64 / FP32 / FP16-offset)")
