@@ -834,8 +834,15 @@ __device__ __forceinline__ void collide_kbc(R (&f)[27], const Moments<R>& M, R d
     }
     const R beta = Fm<R>::rcp(R(6) * nu + R(1));
     const R ib = R(6) * nu + R(1);
-    // 0/0 only when dh == 0 everywhere, where any finite gamma gives the same f
-    const R gamma = den != R(0) ? ib - (R(2) - ib) * Fm<R>::div(num, den) : R(2);
+    // 0/0 only when dh == 0 everywhere, where any finite gamma gives the same
+    // f.  FP32 divides through the hardware reciprocal, which overflows to
+    // infinity for a subnormal den (below 2^-126): with the exact deviations
+    // of FP32 storage, a quiescent cell's dh can fall under 1e-19, and the
+    // infinite gamma times a zero dh made NaNs that spread over the whole case
+    // (KBC, 512x384x256 pulsatile junction, within 40,000 steps).  Such a cell
+    // takes gamma = 2 as well; its dh term is then below 1e-19
+    const bool den_ok = sizeof(R) == 4 ? fabs(den) >= R(1.17549435e-38f) : den != R(0);
+    const R gamma = den_ok ? ib - (R(2) - ib) * Fm<R>::div(num, den) : R(2);
     const R src = (R(1) - beta) * R(3) * Fm<R>::rcp(rho);
     const R uF = ux * Fx + uy * Fy + uz * Fz;
     if constexpr (sizeof(R) == 8) {

@@ -179,3 +179,36 @@ def test_half_storage_populations_against_oracle(gpu_lib, storage, atol, make):
         cpu.step(n - done)
         done = n
         assert_populations_match(gpu.populations(), cpu.populations(), act, done, atol, balanced=True)
+
+
+def test_kbc_fp32_subnormal_entropic_sum_stays_finite(gpu_lib, tmp_path):
+    """FP32 KBC where every stored deviation f - w is below 3e-22, so that the
+    entropic sum sum_q dh_q^2 / f_eq,q is an FP32 subnormal (about 1e-40).
+    The hardware reciprocal of a subnormal overflows to infinity, and dividing
+    by it gave NaN populations that spread over the case (seen after tens of
+    thousands of steps on the 512x384x256 pulsatile junction, where quiescent
+    cells reach such deviations).  The 64x32x32 channel at rest, its
+    checkpointed FP32 words replaced by random deviations of at most 3e-22,
+    two steps (one of each AA parity): every held population must be finite
+    and equal, to FP32 rounding (1e-12), the channel stepped from the exact
+    rest state (whose deviations are the FP32 weights' rounding corrections,
+    about 1e-8)."""
+    from tests.helpers import stored_populations
+    g, kw = channel_case(_abi.F32, _abi.KBC, ivc_umax=0.0)
+    rest = Case(g.flat(), g.dims, **kw)
+    rest.save_state(tmp_path)
+    rest.step(2)
+    p0 = rest.populations()
+    rest.close()
+    words = tmp_path / "df_dev_rank000"
+    n = np.fromfile(words, dtype=np.float32).size
+    np.random.default_rng(5).uniform(-3e-22, 3e-22, n).astype(np.float32).tofile(words)
+    c = Case(g.flat(), g.dims, **kw)
+    c.load_state(tmp_path)
+    c.step(2)
+    act = c.field("label") != 1
+    held = stored_populations(act, 2)
+    p = c.populations()[:, act][held]
+    c.close()
+    assert np.isfinite(p).all(), f"{int((~np.isfinite(p)).sum())} non-finite populations"
+    assert np.abs(p - p0[:, act][held]).max() < 1e-12
