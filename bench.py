@@ -615,8 +615,9 @@ def main() -> int:
     mlups = active * args.steps / wall / 1e6
     print(f"# timed {args.steps} steps in {wall:.3f} s -> {mlups:.1f} fluid MLUPS", file=sys.stderr, flush=True)
 
-    # avg_step_ms: HIP events on the case's stream around 1024 back-to-back
-    # steps, one k_step launch each; kern_ms: an event pair around each launch
+    # avg_step_ms: HIP events on the case's stream around the first 512 of
+    # 1024 steps, launched back to back (32 whole statistics groups), one
+    # k_step launch each; kern_ms: an event pair around each of the other 512
     step_ms, kern_ms = case.time_kernel(1024)
     rho = case.field("rho")
     vx = case.field("vx")
@@ -701,7 +702,7 @@ def main() -> int:
         i2 = c2.info()
         bpg2, nt2, cap2 = configure(c2, args.launch_hbm)
         c2.step(50)
-        s2, k2 = c2.time_kernel(400)
+        s2, k2 = c2.time_kernel(512)  # 256 back-to-back steps: 16 whole statistics groups
         b2 = i2["active_cells"] * 2 * 27 * sbytes
         a2 = b2 / (s2 * 1e-3) / 1e9
         out["roofline_hbm_resident"] = {

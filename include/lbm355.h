@@ -421,9 +421,10 @@ int lbm355_case_labels(const lbm355_case_desc* desc, int32_t* labels, lbm355_cas
 
 /* --- raw kernel timing (measurement only) ---------------------------------- */
 /* n more time steps, timed with HIP events on the case's stream: avg_step_ms =
-   whole step (the fused kernel and its launch gaps) from one event pair around the n
-   steps; avg_kernel_ms = the fused stream-collide launch alone, from an event
-   pair around each launch of the first min(n, 512) steps.  No counterpart in
+   whole step (the fused kernel and its launch gaps) from one event pair around the
+   first (n + 1) / 2 steps, launched back to back; avg_kernel_ms = the fused
+   stream-collide launch alone, from an event pair around each of the next
+   min(n / 2, 512) launches (the rest of the n steps run untimed).  No counterpart in
    the reference, which only logs wall-clock MLUPS (core.h:1440-1469). */
 int lbm355_case_time_kernel(lbm355_case* c, int64_t n, double* avg_step_ms, double* avg_kernel_ms);
 

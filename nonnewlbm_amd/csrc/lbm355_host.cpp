@@ -2310,9 +2310,9 @@ int lbm355_case_autotune(lbm355_case* c, int* best)
 {
     if (!c) return fail(LBM355_EINVAL, "null case");
     // kTuneWarmup steps first; round 1: every configuration on 2 warm-up steps
-    // and a timed window W (tune_window); round 2: the kTuneFinalists fastest,
-    // kTunePasses interleaved passes of 2 + 4W steps, each keeping its fastest
-    // pass.  Single windows of a ~20 us step are within launch jitter and
+    // and a timed window of W back-to-back steps (tune_window); round 2: the
+    // kTuneFinalists fastest, kTunePasses interleaved passes timing 4W steps,
+    // each keeping its fastest pass.  Single windows of a ~20 us step are within launch jitter and
     // clock drift of each other: with 3 finalists and two passes of 2W steps
     // the 256x192x128 junction settled on 4 bricks per workgroup and a cap of
     // 16 in one bench run (19.45 us per step) against 18.4 us for 1 brick.
@@ -2327,7 +2327,14 @@ int lbm355_case_autotune(lbm355_case* c, int* best)
         c->waves_cap = t.cap;
         if ((rc = lbm355_case_step(c, 2))) return rc;  // warm-up
         double step_ms = 0.0;
-        if ((rc = lbm355_case_time_kernel(c, W, &step_ms, nullptr))) return rc;
+        // time_kernel(2n) times its first n steps back to back: W steps, one
+        // whole statistics group.  Timing only W/2 put the group's flush step
+        // (about twice a plain step) into the window of every other
+        // configuration; with 18 steps per configuration that phase repeats
+        // every 8, so the same four shapes carried the flush in every cap
+        // pass and lost round 1 (512x384x256 KBC: the three fastest shapes,
+        // 145-148 us per step, missed the finalists and 157 us was kept)
+        if ((rc = lbm355_case_time_kernel(c, 2 * W, &step_ms, nullptr))) return rc;
         round1.push_back({step_ms, k});
     }
     std::sort(round1.begin(), round1.end());
@@ -2341,7 +2348,7 @@ int lbm355_case_autotune(lbm355_case* c, int* best)
             c->waves_cap = t.cap;
             if ((rc = lbm355_case_step(c, 2))) return rc;
             double step_ms = 0.0;
-            if ((rc = lbm355_case_time_kernel(c, 4 * W, &step_ms, nullptr))) return rc;
+            if ((rc = lbm355_case_time_kernel(c, 8 * W, &step_ms, nullptr))) return rc;  // 4W back to back
             if (pass == 0 || step_ms < fin[size_t(r)]) fin[size_t(r)] = step_ms;
         }
     int best_r = 0;
