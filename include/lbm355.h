@@ -325,9 +325,11 @@ int lbm355_batch_set_group_bytes(lbm355_batch* b, int64_t bytes);
    *best = bricks_per_group | nontemporal << 8 | waves_per_cu << 16 |
            (group bytes >> 24) << 24 (lbm355_batch_config gives the rest) */
 int lbm355_batch_autotune(lbm355_batch* b, int* best);
-/* the same for one case without grouping; instead the 4 fastest
-   configurations are timed again, in three interleaved passes of 2 + 4W
-   steps, each keeping its fastest pass (64 + 24 (2 + W) + 12 (2 + 4W) steps) */
+/* the same for one case without grouping; each configuration is timed over W
+   back-to-back steps (2 warm-up steps, then lbm355_case_time_kernel of 2W),
+   then the 4 fastest again, in three interleaved passes timing 4W steps
+   (2 + 8W each), each keeping its fastest pass (64 + 24 (2 + 2W) + 12 (2 + 8W)
+   steps in all) */
 int lbm355_case_autotune(lbm355_case* c, int* best);
 
 /* --- serving pool ----------------------------------------------------------
