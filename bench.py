@@ -24,7 +24,9 @@ requests of the 200-request serving stream (Poisson arrivals at 2/s, a
 the storage precision study: the junction scaled to 512x384x256 with
 pulsatile inflow (A = 0.3, T = 4,000 steps), cumulant + CY-A, 40,000 steps
 stored as FP64, FP32, FP16- and BF16-offset, each field's error against the
-FP64 run (about 30 s of GPU time).  The
+FP64 run (about 30 s of GPU time); and each collision model at FP32 against
+its FP64 run after 20,000 steps of the 256x192x128 junction (cfg1_parity).
+The
 CPU baseline is the oracle (a dense A/B OpenMP restatement of the
 reference's Host build) on --cpu-steps steps of the benchmark case, with the
 thread count used and lscpu's physical core count.
@@ -354,26 +356,28 @@ def models(steps: int) -> dict:
     return out
 
 
-def precision_study(scale: float, steps: int, collision: int | None = None, storages=None) -> dict:
+def precision_study(scale: float, steps: int, collision: int | None = None, storages=None,
+                    pulse_amp: float = 0.3) -> dict:
     """Storage precision against FP64 on the pulsatile junction: cumulant (or
     ``collision``) + CY-A (neq), parabolic inflows 0.04/0.03 with u(t) = u (1 +
-    0.3 sin(2 pi t / 4000)), FP64 statistics.  Relative L2 over the non-wall
-    cells of u and of the output-path gamma_dot and wss_proxy (the VTK
-    formulas, sim_tcpc.cu:697-714), and the LPA share of the mean outlet
-    flow."""
+    0.3 sin(2 pi t / 4000)) (steady with pulse_amp = 0), FP64 statistics.
+    Relative L2 over the non-wall cells of u and of the output-path gamma_dot
+    and wss_proxy (the VTK formulas, sim_tcpc.cu:697-714), and the LPA share
+    of the mean outlet flow."""
     from nonnewlbm_amd import _abi
     from nonnewlbm_amd.lbm355 import Case
     from nonnewlbm_amd.materials import material_by_name
     mat = material_by_name("cy-a")
     model = _abi.CUM if collision is None else collision
     name = {v: k for k, v in _abi.COLLISIONS.items() if k in ("cumulant", "bgk", "mrt_les", "kbc_n4")}[model]
-    res: dict = {"workload": f"TCPC junction x{scale} pulsatile (A=0.3, T=4000 steps), {name} + CY-A neq, "
-                             f"{steps} steps", "bar": "u rel L2 <= 1e-3 and LPA split within 0.5 % of FP64",
-                 "storages": {}}
+    inflow = f"pulsatile (A={pulse_amp}, T=4000 steps)" if pulse_amp else "steady parabolic inflow"
+    res: dict = {"workload": f"TCPC junction x{scale} {inflow}, {name} + CY-A neq, {steps} steps",
+                 "bar": "u rel L2 <= 1e-3 and LPA split within 0.5 % of FP64", "storages": {}}
     ref = None
     for st in storages or (_abi.F64, _abi.F32, _abi.F16_OFFSET, _abi.BF16_OFFSET):
         g, kw = make_case(scale, st, model, 0.01)
-        kw.update(pulse_amp=0.3, pulse_period=4000.0)
+        if pulse_amp:
+            kw.update(pulse_amp=pulse_amp, pulse_period=4000.0)
         c = Case(g.flat(), g.dims, **kw)
         t0 = time.perf_counter()
         c.step(steps)
@@ -414,6 +418,23 @@ def precision_study(scale: float, steps: int, collision: int | None = None, stor
         res["storages"][_abi.STORAGE_NAMES[st]] = row
         print(f"# precision {name} {_abi.STORAGE_NAMES[st]}: {row}", file=sys.stderr, flush=True)
     return res
+
+
+def cfg1_parity(steps: int) -> dict:
+    """The 256x192x128 junction (steady parabolic inflows 0.04/0.03, CY-A
+    neq) for ``steps`` steps under each collision model, FP32 storage against
+    the FP64 run of the same model on the GPU.  The FP64 path is the one the
+    parity tests hold to the CPU oracle (populations to 1e-12); the oracle
+    itself would take about 40 minutes per model at this size."""
+    from nonnewlbm_amd import _abi
+    out = {}
+    for name, m in (("cumulant", _abi.CUM), ("bgk", _abi.BGK), ("mrt_les", _abi.MRT), ("kbc_n4", _abi.KBC)):
+        r = precision_study(1.0, steps, m, (_abi.F64, _abi.F32), pulse_amp=0.0)
+        f32 = r["storages"]["f32"]
+        out[name] = {k: f32[k] for k in ("u_rel_l2", "rho_minus_1_rel_l2", "gamma_dot_rel_l2", "wss_rel_l2",
+                                         "split_abs_diff_percent", "meets_bar")}
+    return {"workload": f"TCPC junction 256x192x128, steady parabolic inflow 0.04/0.03, CY-A neq, {steps} steps, "
+                        "FP32 against FP64 per collision model (GPU)", "bar": "u rel L2 <= 1e-3", "models": out}
 
 
 def configure(case, launch: str) -> tuple[int, int, int]:
@@ -560,6 +581,8 @@ def main() -> int:
                     help="steps of the FP64/FP32/FP16/BF16 storage study on the 512x384x256 pulsatile junction "
                          "(0: skip)")
     ap.add_argument("--precision-scale", type=float, default=2.0)
+    ap.add_argument("--cfg1-steps", type=int, default=20000,
+                    help="steps of the FP32-against-FP64 check of every collision model at 256x192x128 (0: skip)")
     ap.add_argument("--precision-models", type=int, default=0, metavar="STEPS",
                     help="the same study for BGK, MRT_LES and KBC_N4 (FP64 / FP32 / FP16-offset)")
     ap.add_argument("--launch", default="auto", metavar="BPG,NT,CAP",
@@ -730,6 +753,8 @@ def main() -> int:
 
     if args.precision_study > 0:
         out["precision_study"] = precision_study(args.precision_scale, args.precision_study)
+    if args.cfg1_steps > 0:
+        out["cfg1_parity"] = cfg1_parity(args.cfg1_steps)
     if args.precision_models > 0:
         from nonnewlbm_amd import _abi
         out["precision_models"] = {

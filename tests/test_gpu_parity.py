@@ -341,6 +341,24 @@ def test_inflow_hook_mid_run(gpu_lib, storage):
 
 
 @pytest.mark.parametrize("collision", [_abi.CUM, _abi.BGK, _abi.MRT, _abi.KBC])
+def test_tcpc_models_fp32_full_size_against_fp64(gpu_lib, collision):
+    """The junction workload at its stated size and length: 256x192x128,
+    20,000 steps of each collision model, FP32 storage against the FP64 path
+    (the one held to the CPU oracle to 1e-12 on the smaller cases here; the
+    oracle would need about 40 minutes per model at this size).  Steady
+    parabolic inflows 0.04/0.03, Carreau-Yasuda set A (neq), nu_lbm = 0.01.
+    Bar: 1e-3 relative L2 of velocity and of rho - 1 over the non-wall cells,
+    flow split within 0.5 share points."""
+    import bench
+    r = bench.precision_study(1.0, 20000, collision, (_abi.F64, _abi.F32), pulse_amp=0.0)
+    f32 = r["storages"]["f32"]
+    print(f"model {collision}: {f32}")
+    assert np.isfinite(f32["u_rel_l2"]) and f32["u_rel_l2"] < 1e-3
+    assert f32["rho_minus_1_rel_l2"] < 1e-3
+    assert f32["split_abs_diff_percent"] < 0.5
+
+
+@pytest.mark.parametrize("collision", [_abi.CUM, _abi.BGK, _abi.MRT, _abi.KBC])
 def test_tcpc_models_fp32_20000_steps(gpu_lib, collision):
     """The junction workload's stated length: 20,000 steps of each collision
     model at FP32 storage against the FP64 oracle, on the junction at scale
