@@ -2309,15 +2309,15 @@ int lbm355_batch_autotune(lbm355_batch* b, int* best)
 int lbm355_case_autotune(lbm355_case* c, int* best)
 {
     if (!c) return fail(LBM355_EINVAL, "null case");
-    // kTuneWarmup steps first; round 1: every configuration on 2 warm-up steps
-    // and a timed window of W back-to-back steps (tune_window); round 2: the
+    // kTuneWarmup steps first; stage 1: every configuration on 2 warm-up steps
+    // and a timed window of W back-to-back steps (tune_window); stage 2: the
     // kTuneFinalists fastest, kTunePasses interleaved passes timing 4W steps,
     // each keeping its fastest pass.  Single windows of a ~20 us step are within launch jitter and
     // clock drift of each other: with 3 finalists and two passes of 2W steps
     // the 256x192x128 junction settled on 4 bricks per workgroup and a cap of
     // 16 in one bench run (19.45 us per step) against 18.4 us for 1 brick.
     const int W = tune_window(c->d.stats_every);
-    std::vector<std::pair<double, int>> round1;
+    std::vector<std::pair<double, int>> stage1;
     int rc;
     if ((rc = lbm355_case_step(c, kTuneWarmup))) return rc;
     for (int k = 0; k < kTuneConfigs; ++k) {
@@ -2332,17 +2332,17 @@ int lbm355_case_autotune(lbm355_case* c, int* best)
         // (about twice a plain step) into the window of every other
         // configuration; with 18 steps per configuration that phase repeats
         // every 8, so the same four shapes carried the flush in every cap
-        // pass and lost round 1 (512x384x256 KBC: the three fastest shapes,
+        // pass and lost stage 1 (512x384x256 KBC: the three fastest shapes,
         // 145-148 us per step, missed the finalists and 157 us was kept)
         if ((rc = lbm355_case_time_kernel(c, 2 * W, &step_ms, nullptr))) return rc;
-        round1.push_back({step_ms, k});
+        stage1.push_back({step_ms, k});
     }
-    std::sort(round1.begin(), round1.end());
-    const int nf = std::min(kTuneFinalists, int(round1.size()));
+    std::sort(stage1.begin(), stage1.end());
+    const int nf = std::min(kTuneFinalists, int(stage1.size()));
     std::vector<double> fin(size_t(nf), 0.0);
     for (int pass = 0; pass < kTunePasses; ++pass)
         for (int r = 0; r < nf; ++r) {
-            const TuneConfig t = tune_config(round1[size_t(r)].second);
+            const TuneConfig t = tune_config(stage1[size_t(r)].second);
             c->bricks_per_group = t.bpg;
             c->nontemporal = t.nt;
             c->waves_cap = t.cap;
@@ -2354,7 +2354,7 @@ int lbm355_case_autotune(lbm355_case* c, int* best)
     int best_r = 0;
     for (int r = 1; r < nf; ++r)
         if (fin[size_t(r)] < fin[size_t(best_r)]) best_r = r;
-    const TuneConfig bt = tune_config(round1[size_t(best_r)].second);
+    const TuneConfig bt = tune_config(stage1[size_t(best_r)].second);
     const int best_bpg = bt.bpg, best_nt = bt.nt, best_cap = bt.cap;
     c->bricks_per_group = best_bpg;
     c->nontemporal = best_nt;
